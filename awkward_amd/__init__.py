@@ -1,3 +1,28 @@
 """awkward_amd -- an MI355X-native ``hip`` backend for Awkward Array's
 jagged-array kernel layer.
+
+The kernels are hand-written HIP for gfx950 in ``libawkward_hip.so``
+(``awkward_amd/csrc``), served under the reference's kernel names through
+``HipBackend`` (``backend[(name, *dtypes)](*args)``), and driven by a host
+layer that issues the reference's call sequences for ``sum``, ``max``,
+``min``, ``argmax``, ``argmin``, ``count``, ``num``, ``combinations``,
+jagged and range ``__getitem__``, ``sort`` and ``argsort`` at axis=-1.
 """
+
+from awkward_amd.highlevel import (  # noqa: F401
+    Array,
+    argmax,
+    argmin,
+    argsort,
+    combinations,
+    count,
+    from_list,
+    from_offsets,
+    local_index,
+    max,
+    min,
+    num,
+    sort,
+    sum,
+    to_backend,
+)

@@ -1,0 +1,39 @@
+"""Loads libawkward_hip.so, the C-ABI library of hand-written gfx950 kernels.
+
+There is no fallback: if the library has not been built, or no HIP device is
+visible, every kernel call raises.  Build it with ``python -m awkward_amd.build``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libawkward_hip.so")
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise LibraryMissing(
+                f"{LIBPATH} is not built; run `python -m awkward_amd.build` "
+                "(the hip backend has no CPU fallback)"
+            )
+        handle = ctypes.CDLL(LIBPATH)
+        handle.awkward_hip_set_stream.argtypes = [ctypes.c_void_p]
+        handle.awkward_hip_set_stream.restype = None
+        handle.awkward_hip_set_deferred_errors.argtypes = [ctypes.c_int]
+        handle.awkward_hip_set_deferred_errors.restype = None
+        handle.awkward_hip_version.restype = ctypes.c_char_p
+        handle.awkward_hip_reserve_scratch.argtypes = [ctypes.c_int64]
+        handle.awkward_hip_reserve_scratch.restype = ctypes.c_int
+        _lib = handle
+    return _lib

@@ -1,0 +1,516 @@
+"""Layout nodes of the jagged-array path and the kernel call sequences they
+issue, following the reference's layout code:
+
+* ``ListOffsetArray`` -- ``src/awkward/contents/listoffsetarray.py``
+  (``_reduce_next`` local branch :1394-1430, ``_combinations`` :1152-1261,
+  ``_argsort_next`` / ``_sort_next`` local branches, ``_local_index`` :664-697,
+  ``_carry`` :380-391);
+* ``ListArray`` -- ``contents/listarray.py`` (``_getitem_next_jagged`` NumpyArray
+  branch :549-618, range slice :782-870, ``_compact_offsets64`` :409-433);
+* ``NumpyArray`` -- ``contents/numpyarray.py`` (``_carry`` :367-373,
+  ``_reduce_next`` :1046-1114, ``_argsort_next`` :904-991, ``_sort_next`` :993-1032);
+* ``RecordArray`` -- ``contents/recordarray.py`` (lazy ``_carry`` :552-591);
+* ``IndexedArray`` -- ``contents/indexedarray.py`` (``_reduce_next`` :802-825);
+* ``ByteMaskedArray`` -- the masked output of max/min/argmax/argmin.
+
+Each node holds its backend; every buffer is an array of that backend's
+nplike, and every data-parallel step is a registry kernel or an nplike
+operation of the backend.  Only the axis=-1 paths of the configs are
+implemented; everything else raises NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from awkward_amd import reducers as _reducers
+
+
+class Content:
+    backend = None
+
+    @property
+    def nplike(self):
+        return self.backend.nplike
+
+    def _kernel(self, *key):
+        return self.backend[key]
+
+    def _check(self, err):
+        self.backend.maybe_kernel_error(err)
+
+    def to_list(self):
+        raise NotImplementedError
+
+    @property
+    def is_list(self):
+        return False
+
+
+# ---- leaves ---------------------------------------------------------------------
+class NumpyArray(Content):
+    def __init__(self, data, backend):
+        self.data = data
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self.nplike.length(self.data)
+
+    @property
+    def dtype(self):
+        return self.nplike.dtype(self.data)
+
+    def __repr__(self):
+        return f"NumpyArray(len={self.length}, dtype={self.dtype})"
+
+    def slice(self, start, stop):
+        return NumpyArray(self.data[start:stop], self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        return NumpyArray(self.nplike.take(self.data, carry), self.backend)
+
+    def project(self):
+        return self
+
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims):
+        out = reducer.apply(self, offsets, starts, None, outlength)
+        if mask:
+            outmask = self.nplike.empty(outlength, np.int8)
+            self._check(
+                self._kernel("awkward_NumpyArray_reduce_mask_ByteMaskedArray_64", np.int8, np.int64)(
+                    outmask, offsets, outlength
+                )
+            )
+            out = ByteMaskedArray(outmask, out, False, self.backend)
+        if keepdims:
+            out = RegularArray(out, 1, outlength, self.backend)
+        return out
+
+    def _argsort_next(self, offsets, ascending, stable):
+        nextcarry = self.nplike.empty(self.length, np.int64)
+        self._check(
+            self._kernel("awkward_argsort", np.int64, self.dtype, self.nplike.dtype(offsets))(
+                nextcarry, self.data, self.length, offsets, self.nplike.length(offsets), ascending, stable
+            )
+        )
+        return NumpyArray(nextcarry, self.backend)
+
+    def _sort_next(self, offsets, ascending, stable):
+        out = self.nplike.empty(self.length, self.dtype)
+        self._check(
+            self._kernel("awkward_sort", self.dtype, self.dtype, self.nplike.dtype(offsets))(
+                out, self.data, self.length, offsets, self.nplike.length(offsets), ascending, stable
+            )
+        )
+        return NumpyArray(out, self.backend)
+
+    def to_list(self):
+        return self.nplike.to_numpy(self.data).tolist()
+
+
+class RecordArray(Content):
+    def __init__(self, contents, fields, length, backend):
+        self.contents = list(contents)
+        self.fields = None if fields is None else list(fields)
+        self._length = length
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self._length
+
+    def __repr__(self):
+        return f"RecordArray(fields={self.fields}, len={self.length})"
+
+    def field_names(self):
+        return self.fields if self.fields is not None else [str(i) for i in range(len(self.contents))]
+
+    def _getitem_field(self, where):
+        names = self.field_names()
+        return self.contents[names.index(str(where))]
+
+    def slice(self, start, stop):
+        return RecordArray([c.slice(start, stop) for c in self.contents], self.fields,
+                           max(0, min(stop, self.length) - start), self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        if allow_lazy:
+            return IndexedArray(carry, self, self.backend)
+        return RecordArray([c._carry(carry, False) for c in self.contents], self.fields,
+                           self.nplike.length(carry), self.backend)
+
+    def project(self):
+        return self
+
+    def to_list(self):
+        cols = [c.to_list() for c in self.contents]
+        names = self.field_names()
+        if self.fields is None:
+            return [tuple(col[i] for col in cols) for i in range(self.length)]
+        return [{k: col[i] for k, col in zip(names, cols)} for i in range(self.length)]
+
+
+class IndexedArray(Content):
+    """A lazy carry: element i is content[index[i]]."""
+
+    def __init__(self, index, content, backend):
+        self.index = index
+        self.content = content
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self.nplike.length(self.index)
+
+    def __repr__(self):
+        return f"IndexedArray(len={self.length}, {self.content!r})"
+
+    def slice(self, start, stop):
+        return IndexedArray(self.index[start:stop], self.content, self.backend)
+
+    def _getitem_field(self, where):
+        return IndexedArray(self.index, self.content._getitem_field(where), self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        return IndexedArray(self.nplike.take(self.index, carry), self.content, self.backend)
+
+    def project(self):
+        return self.content._carry(self.index, False)
+
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims):
+        return self.project()._reduce_next(reducer, starts, offsets, outlength, mask, keepdims)
+
+    def _argsort_next(self, offsets, ascending, stable):
+        return self.project()._argsort_next(offsets, ascending, stable)
+
+    def _sort_next(self, offsets, ascending, stable):
+        return self.project()._sort_next(offsets, ascending, stable)
+
+    @property
+    def dtype(self):
+        return self.content.dtype
+
+    @property
+    def data(self):
+        return self.project().data
+
+    def to_list(self):
+        return self.project().to_list()
+
+
+class ByteMaskedArray(Content):
+    def __init__(self, mask, content, valid_when, backend):
+        self.mask = mask
+        self.content = content
+        self.valid_when = valid_when
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self.nplike.length(self.mask)
+
+    def __repr__(self):
+        return f"ByteMaskedArray(valid_when={self.valid_when}, {self.content!r})"
+
+    def to_list(self):
+        mask = self.nplike.to_numpy(self.mask).astype(bool)
+        values = self.content.to_list()
+        valid = mask if self.valid_when else ~mask
+        return [v if ok else None for v, ok in zip(values, valid)]
+
+
+class RegularArray(Content):
+    def __init__(self, content, size, length, backend):
+        self.content = content
+        self.size = size
+        self._length = length
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self._length
+
+    def to_list(self):
+        flat = self.content.to_list()
+        return [flat[i * self.size:(i + 1) * self.size] for i in range(self.length)]
+
+
+# ---- lists ------------------------------------------------------------------------
+class ListOffsetArray(Content):
+    def __init__(self, offsets, content, backend):
+        self.offsets = offsets
+        self.content = content
+        self.backend = backend
+
+    @property
+    def is_list(self):
+        return True
+
+    @property
+    def length(self):
+        return self.nplike.length(self.offsets) - 1
+
+    @property
+    def starts(self):
+        return self.offsets[:-1]
+
+    @property
+    def stops(self):
+        return self.offsets[1:]
+
+    def __repr__(self):
+        return f"ListOffsetArray(len={self.length}, {self.content!r})"
+
+    def _getitem_field(self, where):
+        return ListOffsetArray(self.offsets, self.content._getitem_field(where), self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        return ListArray(self.nplike.take(self.starts.contiguous() if hasattr(self.starts, "contiguous") else self.starts, carry),
+                         self.nplike.take(self.stops.contiguous() if hasattr(self.stops, "contiguous") else self.stops, carry),
+                         self.content, self.backend)
+
+    def to_ListArray(self):
+        return ListArray(self.starts, self.stops, self.content, self.backend)
+
+    def _zero_based(self):
+        """(offsets starting at 0, content trimmed to them): the reference's
+        to_ListOffsetArray64(True) / trim step (one host read of offsets[0])."""
+        nplike = self.nplike
+        offsets = self.offsets
+        if nplike.dtype(offsets) != np.dtype(np.int64):
+            offsets = nplike.asarray(offsets, np.int64)
+        first = nplike.item(offsets, 0)
+        last = nplike.item(offsets, nplike.length(offsets) - 1)
+        if first != 0:
+            offsets = offsets - first
+        return offsets, self.content.slice(first, last)
+
+    # -- reduce, axis=-1: listoffsetarray.py:1394-1430 --
+    def _reduce_local(self, reducer, mask, keepdims):
+        nextoffsets, trimmed = self._zero_based()
+        nextstarts = nextoffsets[:-1]
+        return trimmed._reduce_next(reducer, nextstarts, nextoffsets, self.length, mask, keepdims)
+
+    # -- argsort / sort, axis=-1 --
+    def _argsort_local(self, ascending, stable):
+        nextoffsets, trimmed = self._zero_based()
+        out = trimmed._argsort_next(nextoffsets, ascending, stable)
+        return ListOffsetArray(nextoffsets, out, self.backend)
+
+    def _sort_local(self, ascending, stable):
+        nextoffsets, trimmed = self._zero_based()
+        out = trimmed._sort_next(nextoffsets, ascending, stable)
+        return ListOffsetArray(nextoffsets, out, self.backend)
+
+    # -- local index: listoffsetarray.py:664-697 --
+    def _local_index(self):
+        nplike = self.nplike
+        offsets = self.offsets
+        if nplike.dtype(offsets) != np.dtype(np.int64):
+            offsets = nplike.asarray(offsets, np.int64)
+        innerlength = nplike.item(offsets, nplike.length(offsets) - 1)
+        localindex = nplike.empty(innerlength, np.int64)
+        self._check(
+            self._kernel("awkward_ListArray_localindex", np.int64, np.int64)(localindex, offsets, self.length)
+        )
+        return ListOffsetArray(offsets, NumpyArray(localindex, self.backend), self.backend)
+
+    # -- combinations, axis=1: listoffsetarray.py:1152-1261 --
+    def _combinations(self, n, replacement, fields):
+        nplike = self.nplike
+        starts, stops = self.starts, self.stops
+        if hasattr(starts, "contiguous"):
+            starts, stops = starts.contiguous(), stops.contiguous()
+        totallen_buf = nplike.empty(1, np.int64)
+        offsets = nplike.empty(self.length + 1, np.int64)
+        cdt = nplike.dtype(starts)
+        self._check(
+            self._kernel("awkward_ListArray_combinations_length", np.int64, np.int64, cdt, cdt)(
+                totallen_buf, offsets, n, replacement, starts, stops, self.length
+            )
+        )
+        totallen = nplike.index_as_shape_item(totallen_buf)
+        tocarry = [nplike.empty(totallen, np.int64) for _ in range(n)]
+        toindex = nplike.empty(n, np.int64)
+        fromindex = nplike.empty(n, np.int64)
+        self._check(
+            self._kernel("awkward_ListArray_combinations", np.int64, np.int64, np.int64, cdt, cdt)(
+                self.backend.pointer_array(tocarry), toindex, fromindex, n, replacement, starts, stops, self.length
+            )
+        )
+        contents = [self.content._carry(ptr, True) for ptr in tocarry]
+        record = RecordArray(contents, fields, totallen, self.backend)
+        return ListOffsetArray(offsets, record, self.backend)
+
+    def _getitem_range(self, start, stop, step):
+        return self.to_ListArray()._getitem_range(start, stop, step)
+
+    def _getitem_jagged(self, item):
+        return self.to_ListArray()._getitem_jagged(item)
+
+    def to_list(self):
+        offs = self.nplike.to_numpy(self.offsets)
+        flat = self.content.to_list()
+        return [flat[offs[i]:offs[i + 1]] for i in range(len(offs) - 1)]
+
+
+class ListArray(Content):
+    def __init__(self, starts, stops, content, backend):
+        self.starts = starts
+        self.stops = stops
+        self.content = content
+        self.backend = backend
+
+    @property
+    def is_list(self):
+        return True
+
+    @property
+    def length(self):
+        return self.nplike.length(self.starts)
+
+    def __repr__(self):
+        return f"ListArray(len={self.length}, {self.content!r})"
+
+    def _contig(self):
+        s, e = self.starts, self.stops
+        if hasattr(s, "contiguous"):
+            s, e = s.contiguous(), e.contiguous()
+        return s, e
+
+    def _getitem_field(self, where):
+        return ListArray(self.starts, self.stops, self.content._getitem_field(where), self.backend)
+
+    # -- [:, start:stop:step]: listarray.py:782-870 --
+    def _getitem_range(self, start, stop, step):
+        nplike = self.nplike
+        step = 1 if step is None else step
+        if step == 0:
+            raise ValueError("slice step cannot be zero")
+        kstart = _kSliceNone if start is None else start
+        kstop = _kSliceNone if stop is None else stop
+        starts, stops = self._contig()
+        cdt = nplike.dtype(starts)
+        lenstarts = self.length
+        carrylength = nplike.empty(1, np.int64)
+        self._check(
+            self._kernel("awkward_ListArray_getitem_next_range_carrylength", np.int64, cdt, cdt)(
+                carrylength, starts, stops, lenstarts, kstart, kstop, step
+            )
+        )
+        nextcarry = nplike.empty(nplike.index_as_shape_item(carrylength), np.int64)
+        nextoffsets = nplike.empty(lenstarts + 1, cdt)
+        self._check(
+            self._kernel("awkward_ListArray_getitem_next_range", cdt, np.int64, cdt, cdt)(
+                nextoffsets, nextcarry, starts, stops, lenstarts, kstart, kstop, step
+            )
+        )
+        return ListOffsetArray(nextoffsets, self.content._carry(nextcarry, True), self.backend)
+
+    # -- jagged integer index, NumpyArray slice content: listarray.py:549-618 --
+    def _getitem_next_jagged(self, slicestarts, slicestops, sliceindex):
+        nplike = self.nplike
+        starts, stops = self._contig()
+        cdt = nplike.dtype(starts)
+        carrylen_buf = nplike.empty(1, np.int64)
+        self._check(
+            self._kernel("awkward_ListArray_getitem_jagged_carrylen", np.int64, np.int64, np.int64)(
+                carrylen_buf, slicestarts, slicestops, nplike.length(slicestarts)
+            )
+        )
+        carrylen = nplike.index_as_shape_item(carrylen_buf)
+        outoffsets = nplike.empty(nplike.length(slicestarts) + 1, np.int64)
+        nextcarry = nplike.empty(carrylen, np.int64)
+        err = self._kernel("awkward_ListArray_getitem_jagged_apply", np.int64, np.int64, np.int64, np.int64,
+                           np.int64, cdt, cdt)(
+            outoffsets, nextcarry, slicestarts, slicestops, nplike.length(slicestarts), sliceindex,
+            nplike.length(sliceindex), starts, stops, self.content.length
+        )
+        if err.str is not None:
+            raise IndexError(self.backend.format_kernel_error(err))
+        return ListOffsetArray(outoffsets, self.content._carry(nextcarry, True), self.backend)
+
+    def _getitem_jagged(self, item):
+        """self[item] for a jagged integer index `item` (a ListOffsetArray of
+        int64 with as many lists as self), as Content._getitem wraps it:
+        RegularArray(self, len, 1) sliced by the jagged head
+        (regulararray.py:731-775 -> awkward_RegularArray_getitem_jagged_expand)."""
+        nplike = self.nplike
+        if item.length != self.length:
+            raise IndexError(
+                f"cannot fit jagged slice with length {item.length} into array with length {self.length}")
+        multistarts = nplike.empty(item.length, np.int64)
+        multistops = nplike.empty(item.length, np.int64)
+        self._check(
+            self._kernel("awkward_RegularArray_getitem_jagged_expand", np.int64, np.int64, np.int64)(
+                multistarts, multistops, item.offsets, item.length, 1
+            )
+        )
+        return self._getitem_next_jagged(multistarts, multistops, item.content.data)
+
+    def to_ListOffsetArray64(self):
+        nplike = self.nplike
+        starts, stops = self._contig()
+        cdt = nplike.dtype(starts)
+        offsets = nplike.empty(self.length + 1, np.int64)
+        self._check(
+            self._kernel("awkward_ListArray_compact_offsets", np.int64, cdt, cdt)(offsets, starts, stops, self.length)
+        )
+        # content of the compacted lists: the ranges [starts[i], stops[i]) in order
+        carry = _ranges_carry(self, offsets)
+        return ListOffsetArray(offsets, self.content._carry(carry, True), self.backend)
+
+    def to_list(self):
+        s = self.nplike.to_numpy(self.starts)
+        e = self.nplike.to_numpy(self.stops)
+        flat = self.content.to_list()
+        return [flat[a:b] for a, b in zip(s, e)]
+
+
+def _ranges_carry(listarray, offsets):
+    """Index of every element of a ListArray in list order (the reference's
+    ListArray._compact_offsets64 + broadcast carry): localindex over the
+    compact offsets plus each list's start."""
+    nplike = listarray.nplike
+    total = nplike.item(offsets, nplike.length(offsets) - 1)
+    local = nplike.empty(total, np.int64)
+    listarray._check(
+        listarray._kernel("awkward_ListArray_localindex", np.int64, np.int64)(local, offsets, listarray.length)
+    )
+    starts, _ = listarray._contig()
+    nextcarry = nplike.empty(total, np.int64)
+    # start of each element's list: gather starts by the parents of `local`
+    parents = nplike.parents(offsets, total)
+    nextcarry = nplike.take(nplike.asarray(starts, np.int64), parents) + local
+    return nextcarry
+
+
+_kSliceNone = 2**63 - 1
+
+
+def reduce(layout, reducer, mask, keepdims):
+    """_do.reduce at axis=-1 (src/awkward/_do.py:225-340) for a list of
+    numbers: returns the reduced content (the outer length-1 level removed)."""
+    if isinstance(layout, ListArray):
+        layout = layout.to_ListOffsetArray64()
+    if not isinstance(layout, ListOffsetArray):
+        raise NotImplementedError("hip reductions are implemented at axis=-1 of a list array")
+    inner = layout.content
+    if isinstance(inner, (ListOffsetArray, ListArray, RecordArray)):
+        raise NotImplementedError("hip reductions are implemented for one level of lists over numbers")
+    return layout._reduce_local(reducer, mask, keepdims)
+
+
+__all__ = [
+    "Content",
+    "NumpyArray",
+    "RecordArray",
+    "IndexedArray",
+    "ByteMaskedArray",
+    "RegularArray",
+    "ListOffsetArray",
+    "ListArray",
+    "reduce",
+    "_reducers",
+]

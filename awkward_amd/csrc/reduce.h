@@ -1,0 +1,306 @@
+// reduce.h -- segmented reductions over int64 offsets (gfx950).
+//
+// One kernel template serves awkward_reduce_{sum,max,min,argmax,argmin} and
+// the bool sums (awkward-cpp/src/cpu-kernels/awkward_reduce_*.cpp).  A
+// workgroup of BLOCK threads owns BLOCK consecutive lists:
+//
+//   * light tile (the lists' elements fit in CAP_BYTES of LDS): the span is
+//     staged with coalesced loads, then each thread reduces its own list
+//     sequentially from LDS with the CPU kernel's own loop -- the result is
+//     bit-identical to awkward-cpp, float sums included;
+//   * heavy tile: lists of at most kShortLen elements are still reduced
+//     sequentially by their thread (from global memory); medium lists
+//     (<= kWaveLen) by one wave each, long lists by the whole workgroup.
+//     Cooperative partials carry (value, index) for extrema so that ties
+//     resolve to the first occurrence exactly as the serial loop does; only
+//     float sums of lists longer than kShortLen in heavy tiles change their
+//     summation order (within the stated tolerance).
+//
+// Ops (struct with the members used below):
+//   In, Out, Part
+//   Out  sequential(const In* p, int64_t b, int64_t e, int64_t gb) -- the CPU loop
+//   Part init(), elem(In v, int64_t gi), combine(Part a, Part b)
+//   Out  finalize(Part p, const In* global, int64_t gb, int64_t ge)
+//   Part shfl_xor(Part p, int mask)
+#pragma once
+
+#include <cmath>
+#include <type_traits>
+
+#include "common.h"
+
+namespace akhip {
+
+constexpr int64_t kShortLen = 32;
+constexpr int64_t kWaveLen = 4096;
+
+// ---- shuffles of arbitrary small trivially-copyable values ----------------
+template <typename T>
+__device__ inline T shfl_xor_any(T v, int mask) {
+  if constexpr (sizeof(T) < 4) {
+    int x = (int)v;
+    x = __shfl_xor(x, mask, kWave);
+    return (T)x;
+  } else if constexpr (sizeof(T) == 4) {
+    int x;
+    __builtin_memcpy(&x, &v, 4);
+    x = __shfl_xor(x, mask, kWave);
+    T out;
+    __builtin_memcpy(&out, &x, 4);
+    return out;
+  } else {
+    static_assert(sizeof(T) % 4 == 0, "shuffle of odd-sized type");
+    int w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = __shfl_xor(w[k], mask, kWave);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+  }
+}
+
+template <typename T>
+__device__ inline bool is_nan(T v) {
+  if constexpr (std::is_floating_point_v<T>) return v != v;
+  else return false;
+}
+
+// Integer adds wrap (two's complement) like the CPU kernels' native adds.
+template <typename OUT>
+__device__ inline OUT wrap_add(OUT a, OUT b) {
+  if constexpr (std::is_floating_point_v<OUT>) {
+    return a + b;
+  } else {
+    using U = std::make_unsigned_t<OUT>;
+    return (OUT)((U)a + (U)b);
+  }
+}
+
+// ---- ops --------------------------------------------------------------------
+template <typename OUT, typename IN>
+struct SumOp {
+  using In = IN;
+  using Out = OUT;
+  using Part = OUT;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    OUT acc = OUT(0);
+    for (int64_t i = b; i < e; i++) acc = wrap_add<OUT>(acc, (OUT)p[i]);
+    return acc;
+  }
+  __device__ Part init() const { return OUT(0); }
+  __device__ Part elem(In v, int64_t) const { return (OUT)v; }
+  __device__ Part combine(Part a, Part b) const { return wrap_add<OUT>(a, b); }
+  __device__ Out finalize(Part p, const In*, int64_t, int64_t) const { return p; }
+  __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_any(p, m); }
+};
+
+template <typename OUT>
+struct SumBoolOp {  // awkward_reduce_sum_{int64,int32}_bool_64: count of true
+  using In = bool;
+  using Out = OUT;
+  using Part = OUT;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    OUT acc = 0;
+    for (int64_t i = b; i < e; i++)
+      if (p[i]) acc++;
+    return acc;
+  }
+  __device__ Part init() const { return 0; }
+  __device__ Part elem(In v, int64_t) const { return v ? 1 : 0; }
+  __device__ Part combine(Part a, Part b) const { return a + b; }
+  __device__ Out finalize(Part p, const In*, int64_t, int64_t) const { return p; }
+  __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_any(p, m); }
+};
+
+template <typename T>
+struct ValIdx {
+  T val;
+  int64_t idx;  // -1: no candidate
+};
+
+// Index-aware extremum combine: ignores missing candidates, prefers the
+// strictly better value, and on equal values (including -0.0 == +0.0) the
+// smaller index -- the element the serial loop would have kept.
+template <typename T, bool MAX>
+__device__ inline ValIdx<T> better(ValIdx<T> a, ValIdx<T> b) {
+  if (a.idx < 0) return b;
+  if (b.idx < 0) return a;
+  const bool b_wins = MAX ? (b.val > a.val) : (b.val < a.val);
+  const bool a_wins = MAX ? (a.val > b.val) : (a.val < b.val);
+  if (b_wins) return b;
+  if (a_wins) return a;
+  return (b.idx < a.idx) ? b : a;
+}
+
+template <typename T>
+__device__ inline ValIdx<T> shfl_xor_vi(ValIdx<T> p, int m) {
+  return ValIdx<T>{shfl_xor_any(p.val, m), shfl_xor_any(p.idx, m)};
+}
+
+// awkward_reduce_max / awkward_reduce_min: best = identity; best = v OP best ? v : best
+template <typename T, bool MAX>
+struct ExtremumOp {
+  using In = T;
+  using Out = T;
+  using Part = ValIdx<T>;
+  T identity;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    T best = identity;
+    for (int64_t i = b; i < e; i++) {
+      const T v = p[i];
+      best = (MAX ? (v > best) : (v < best)) ? v : best;
+    }
+    return best;
+  }
+  __device__ Part init() const { return Part{identity, -1}; }
+  __device__ Part elem(In v, int64_t gi) const { return Part{v, is_nan(v) ? -1 : gi}; }
+  __device__ Part combine(Part a, Part b) const { return better<T, MAX>(a, b); }
+  __device__ Out finalize(Part p, const In*, int64_t, int64_t) const {
+    if (p.idx < 0) return identity;
+    return (MAX ? (p.val > identity) : (p.val < identity)) ? p.val : identity;
+  }
+  __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_vi(p, m); }
+};
+
+// awkward_reduce_argmax / argmin: first element seeds; strict improvement
+// only.  A NaN first element therefore wins; later NaNs never do.
+template <typename T, bool MAX>
+struct ArgOp {
+  using In = T;
+  using Out = int64_t;
+  using Part = ValIdx<T>;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t gb) const {
+    if (b >= e) return -1;
+    int64_t best = b;
+    T best_val = p[b];
+    for (int64_t i = b + 1; i < e; i++) {
+      const T v = p[i];
+      if (MAX ? (v > best_val) : (v < best_val)) {
+        best_val = v;
+        best = i;
+      }
+    }
+    return gb + (best - b);
+  }
+  __device__ Part init() const { return Part{T(0), -1}; }
+  __device__ Part elem(In v, int64_t gi) const { return Part{v, is_nan(v) ? -1 : gi}; }
+  __device__ Part combine(Part a, Part b) const { return better<T, MAX>(a, b); }
+  __device__ Out finalize(Part p, const In* g, int64_t gb, int64_t ge) const {
+    if (gb >= ge) return -1;
+    if (is_nan(g[gb])) return gb;
+    return p.idx;
+  }
+  __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_vi(p, m); }
+};
+
+// ---- the kernel -----------------------------------------------------------------
+// Epilogue applied to each list's result before the store (identity for the
+// plain kernels; the fused host-layer variants subtract the list start and
+// write the empty-list mask).
+struct PlainEpilogue {
+  template <typename Out>
+  __device__ Out apply(Out v, int64_t, int64_t, int64_t) const { return v; }
+};
+
+template <class Op, int BLOCK, int CAP_BYTES, class Epi = PlainEpilogue>
+__global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi, typename Op::Out* __restrict__ toptr,
+                                                                 const typename Op::In* __restrict__ fromptr,
+                                                                 const int64_t* __restrict__ offsets,
+                                                                 int64_t outlength) {
+  using In = typename Op::In;
+  using Out = typename Op::Out;
+  using Part = typename Op::Part;
+  constexpr int CAP = CAP_BYTES / (int)sizeof(In);
+  constexpr int NW = BLOCK / kWave;
+  __shared__ In s_data[CAP];
+  __shared__ int64_t s_off[BLOCK + 1];
+  __shared__ uint64_t s_long[NW];
+  __shared__ Part s_part[NW];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  const int64_t L0 = (int64_t)blockIdx.x * BLOCK;
+  const int nl = (int)((outlength - L0) < BLOCK ? (outlength - L0) : BLOCK);
+
+  for (int t = tid; t <= nl; t += BLOCK) s_off[t] = offsets[L0 + t];
+  __syncthreads();
+  const int64_t E0 = s_off[0];
+  const int64_t span = s_off[nl] - E0;
+
+  if (span <= CAP) {
+    // light tile: stage, then one serial loop per thread
+    constexpr int U = 8;
+    for (int64_t b = 0; b < span; b += (int64_t)BLOCK * U) {
+      In r[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t k = b + (int64_t)u * BLOCK + tid;
+        if (k < span) r[u] = fromptr[E0 + k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t k = b + (int64_t)u * BLOCK + tid;
+        if (k < span) s_data[k] = r[u];
+      }
+    }
+    __syncthreads();
+    if (tid < nl) {
+      const int64_t gb = s_off[tid], ge = s_off[tid + 1];
+      const Out v = op.sequential(s_data, gb - E0, ge - E0, gb);
+      toptr[L0 + tid] = epi.apply(v, L0 + tid, gb, ge);
+    }
+    return;
+  }
+
+  // heavy tile
+  int64_t gb = 0, ge = 0;
+  if (tid < nl) {
+    gb = s_off[tid];
+    ge = s_off[tid + 1];
+    if (ge - gb <= kShortLen) toptr[L0 + tid] = epi.apply(op.sequential(fromptr, gb, ge, gb), L0 + tid, gb, ge);
+  }
+  const int64_t len = ge - gb;
+  // medium lists: the owning wave reduces each one cooperatively
+  uint64_t medium = __ballot(tid < nl && len > kShortLen && len <= kWaveLen);
+  while (medium) {
+    const int b = __builtin_ctzll(medium);
+    medium &= medium - 1;
+    const int owner = wave * kWave + b;
+    const int64_t lb = s_off[owner], le = s_off[owner + 1];
+    Part p = op.init();
+    for (int64_t i = lb + lane; i < le; i += kWave) p = op.combine(p, op.elem(fromptr[i], i));
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
+    if (lane == 0) toptr[L0 + owner] = epi.apply(op.finalize(p, fromptr, lb, le), L0 + owner, lb, le);
+  }
+  // long lists: the whole workgroup
+  const uint64_t lng = __ballot(tid < nl && len > kWaveLen);
+  if (lane == 0) s_long[wave] = lng;
+  __syncthreads();
+  for (int w = 0; w < NW; w++) {
+    uint64_t m = s_long[w];
+    while (m) {
+      const int b = __builtin_ctzll(m);
+      m &= m - 1;
+      const int owner = w * kWave + b;
+      const int64_t lb = s_off[owner], le = s_off[owner + 1];
+      Part p = op.init();
+#pragma unroll 4
+      for (int64_t i = lb + tid; i < le; i += BLOCK) p = op.combine(p, op.elem(fromptr[i], i));
+#pragma unroll
+      for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
+      if (lane == 0) s_part[wave] = p;
+      __syncthreads();
+      if (tid == 0) {
+        Part q = s_part[0];
+        for (int k = 1; k < NW; k++) q = op.combine(q, s_part[k]);
+        toptr[L0 + owner] = epi.apply(op.finalize(q, fromptr, lb, le), L0 + owner, lb, le);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace akhip

@@ -1,0 +1,152 @@
+// scan.h -- single-pass exclusive scan with decoupled look-back (gfx950).
+//
+// Used wherever the CPU kernels build offsets with a running `k`
+// (compact_offsets, combinations_length, getitem_jagged_apply,
+// getitem_next_range, boolean-mask compaction).  One launch reads each input
+// once and writes each output once:
+//
+//   * each workgroup claims a tile id from an atomic counter, so every tile
+//     it waits on has already started (no dependence on dispatch order);
+//   * the tile's items are counted (Body::count), scanned in registers and
+//     across the wave with shuffles, then across waves through LDS;
+//   * the tile publishes its aggregate, and wave 0 looks back over up to 64
+//     predecessors at a time until it meets a published inclusive prefix;
+//   * Body::emit(i, exclusive_prefix, count) writes the outputs.
+//
+// Status words pack a 2-bit flag with a 62-bit non-negative value, so one
+// 64-bit atomic carries both.  The look-back spin is bounded: a tile that
+// waited ~2^26 polls records an error instead of hanging the device.
+#pragma once
+
+#include "common.h"
+
+namespace akhip {
+
+constexpr uint64_t kScanAgg = uint64_t(1) << 62;
+constexpr uint64_t kScanPrefix = uint64_t(2) << 62;
+constexpr uint64_t kScanFlags = uint64_t(3) << 62;
+constexpr uint64_t kScanValue = ~kScanFlags;
+constexpr int kScanTimeoutCode = 250;
+
+struct ScanScratch {
+  unsigned* counter;
+  uint64_t* status;
+};
+
+inline size_t scan_scratch_bytes(int64_t ntiles) { return 256 + (size_t)ntiles * sizeof(uint64_t); }
+
+inline ScanScratch scan_scratch_at(void* base) {
+  return ScanScratch{(unsigned*)base, (uint64_t*)((char*)base + 256)};
+}
+
+// Clears the tile counter and the status words before a scan (a kernel, not
+// hipMemsetAsync, so the sequence can be captured into a hipGraph).
+template <int Dummy = 0>
+__global__ void scan_prep_kernel(ScanScratch s, int64_t ntiles) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *s.counter = 0u;
+  if (i < ntiles) s.status[i] = 0;
+}
+
+template <int BLOCK, int IPT, class Body>
+__global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScratch s, int64_t n,
+                                                              uint64_t* errword, uint32_t inv) {
+  constexpr int NW = BLOCK / kWave;
+  __shared__ int64_t s_wave[NW];
+  __shared__ int64_t s_prefix;
+  __shared__ unsigned s_tile;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+
+  if (tid == 0) s_tile = atomicAdd(s.counter, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * (int64_t)(BLOCK * IPT) + (int64_t)tid * IPT;
+
+  int64_t c[IPT];
+  int64_t total = 0;
+#pragma unroll
+  for (int u = 0; u < IPT; u++) {
+    const int64_t i = base + u;
+    c[u] = (i < n) ? body.count(i) : 0;
+    total += c[u];
+  }
+  const int64_t incl = wave_inclusive_scan(total);
+  if (lane == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  int64_t wave_off = 0, block_total = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    const int64_t v = s_wave[w];
+    wave_off += (w < wave) ? v : 0;
+    block_total += v;
+  }
+
+  if (wave == 0) {
+    int64_t prefix = 0;
+    if (tile == 0) {
+      if (lane == 0)
+        __hip_atomic_store(&s.status[0], kScanPrefix | (uint64_t)block_total, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0)
+        __hip_atomic_store(&s.status[tile], kScanAgg | (uint64_t)block_total, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      int64_t look = tile - 1;
+      bool done = false;
+      while (!done) {
+        const int64_t idx = look - lane;
+        uint64_t w = kScanPrefix;  // lanes before tile 0 read as an empty prefix
+        if (idx >= 0) {
+          uint32_t spins = 0;
+          do {
+            w = __hip_atomic_load(&s.status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if ((w & kScanFlags) != 0) break;
+            __builtin_amdgcn_s_sleep(1);
+          } while (++spins < (1u << 26));
+          if ((w & kScanFlags) == 0) {
+            record_error(errword, inv, tile, kScanTimeoutCode);
+            w = kScanPrefix;
+          }
+        }
+        const uint64_t prefix_lanes = __ballot((w & kScanFlags) == kScanPrefix);
+        int64_t v = (int64_t)(w & kScanValue);
+        if (prefix_lanes != 0) {
+          const int first = __builtin_ctzll(prefix_lanes);
+          v = (lane <= first) ? v : 0;
+          done = true;
+        } else {
+          look -= kWave;
+        }
+        prefix += wave_sum(v);
+      }
+      if (lane == 0)
+        __hip_atomic_store(&s.status[tile], kScanPrefix | (uint64_t)(prefix + block_total),
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  int64_t run = s_prefix + wave_off + incl - total;
+#pragma unroll
+  for (int u = 0; u < IPT; u++) {
+    const int64_t i = base + u;
+    if (i < n) body.emit(i, run, c[u]);
+    run += c[u];
+  }
+}
+
+// Enqueue the scan of n items (n >= 1) on the current stream.
+template <int BLOCK, int IPT, class Body>
+Error launch_lookback_scan(const Body& body, int64_t n, uint32_t inv, void* scratch_base) {
+  const int64_t ntiles = ceil_div(n, (int64_t)BLOCK * IPT);
+  ScanScratch s = scan_scratch_at(scratch_base);
+  hipStream_t st = stream();
+  scan_prep_kernel<0><<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(s, ntiles);
+  lookback_scan_kernel<BLOCK, IPT, Body><<<(unsigned)ntiles, BLOCK, 0, st>>>(body, s, n, error_word(), inv);
+  AKHIP_LAUNCH_CHECK("lookback_scan_kernel");
+  return success();
+}
+
+}  // namespace akhip

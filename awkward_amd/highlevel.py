@@ -1,0 +1,369 @@
+"""User-facing API of the jagged-array path: ``Array`` and the ``ak.*``
+functions the configs use, with the reference's signatures and defaults:
+
+* ``sum`` (``operations/ak_sum.py``: ``mask_identity=False``),
+  ``max`` / ``min`` (``ak_max.py`` / ``ak_min.py``: ``mask_identity=True``,
+  ``initial=None``), ``argmax`` / ``argmin`` (``mask_identity=True``),
+  ``count`` (``mask_identity=False``), ``num`` (``ak_num.py:133-142``);
+* ``combinations`` (``ak_combinations.py``; fields "0".."n-1" by default);
+* ``argsort`` / ``sort`` (``ak_argsort.py`` / ``ak_sort.py``: ascending,
+  ``stable=True``);
+* ``local_index``; ``to_backend``;
+* ``Array.__getitem__`` with a field name, a jagged boolean or integer
+  index, or ``[:, start:stop:step]``; elementwise ufuncs on lists that share
+  offsets.
+
+All of them run at axis=-1 (the innermost list level) of one level of lists.
+"""
+
+from __future__ import annotations
+
+import builtins
+
+import numpy as np
+
+from awkward_amd import contents as C
+from awkward_amd import reducers as R
+
+
+class Array:
+    def __init__(self, layout):
+        self.layout = layout
+
+    # ---- basics -------------------------------------------------------------
+    @property
+    def backend(self):
+        return self.layout.backend
+
+    def __len__(self):
+        return self.layout.length
+
+    def __repr__(self):
+        return f"<Array {self.layout!r}>"
+
+    def to_list(self):
+        return self.layout.to_list()
+
+    tolist = to_list
+
+    @property
+    def fields(self):
+        node = _record_of(self.layout)
+        return [] if node is None else node.field_names()
+
+    def __getattr__(self, where):
+        if where.startswith("_"):
+            raise AttributeError(where)
+        if where in self.fields:
+            return self[where]
+        raise AttributeError(where)
+
+    # ---- slicing --------------------------------------------------------------
+    def __getitem__(self, where):
+        layout = self.layout
+        if isinstance(where, str):
+            return Array(layout._getitem_field(where))
+        if isinstance(where, Array):
+            return Array(_getitem_jagged(layout, where.layout))
+        if isinstance(where, tuple) and len(where) == 2 and where[0] == slice(None):
+            inner = where[1]
+            if isinstance(inner, slice):
+                if not layout.is_list:
+                    raise IndexError("too many dimensions in slice")
+                return Array(layout._getitem_range(inner.start, inner.stop, inner.step))
+        raise NotImplementedError(f"getitem {where!r} on the hip jagged-array path")
+
+    # ---- elementwise ---------------------------------------------------------------
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        if method != "__call__" or kwargs:
+            return NotImplemented
+        return _broadcast_ufunc(ufunc, inputs)
+
+    def _binary(self, ufunc, other, reflected=False):
+        return _broadcast_ufunc(ufunc, (other, self) if reflected else (self, other))
+
+    def __gt__(self, o):
+        return self._binary(np.greater, o)
+
+    def __ge__(self, o):
+        return self._binary(np.greater_equal, o)
+
+    def __lt__(self, o):
+        return self._binary(np.less, o)
+
+    def __le__(self, o):
+        return self._binary(np.less_equal, o)
+
+    def __eq__(self, o):
+        return self._binary(np.equal, o)
+
+    def __ne__(self, o):
+        return self._binary(np.not_equal, o)
+
+    def __and__(self, o):
+        return self._binary(np.bitwise_and, o)
+
+    def __or__(self, o):
+        return self._binary(np.bitwise_or, o)
+
+    def __abs__(self):
+        return _broadcast_ufunc(np.absolute, (self,))
+
+    def __neg__(self):
+        return _broadcast_ufunc(np.negative, (self,))
+
+    def __add__(self, o):
+        return self._binary(np.add, o)
+
+    def __radd__(self, o):
+        return self._binary(np.add, o, True)
+
+    def __sub__(self, o):
+        return self._binary(np.subtract, o)
+
+    def __rsub__(self, o):
+        return self._binary(np.subtract, o, True)
+
+    def __mul__(self, o):
+        return self._binary(np.multiply, o)
+
+    def __rmul__(self, o):
+        return self._binary(np.multiply, o, True)
+
+    __hash__ = None
+
+
+def _record_of(layout):
+    node = layout
+    while isinstance(node, (C.ListOffsetArray, C.ListArray, C.IndexedArray)):
+        node = node.content
+    return node if isinstance(node, C.RecordArray) else None
+
+
+def _leaf(layout):
+    """(list structure, flat data) of a list-of-numbers or flat layout."""
+    if isinstance(layout, C.ListArray):
+        layout = layout.to_ListOffsetArray64()
+    if isinstance(layout, C.ListOffsetArray):
+        inner = layout.content
+        if isinstance(inner, C.IndexedArray):
+            inner = inner.project()
+        if not isinstance(inner, C.NumpyArray):
+            raise NotImplementedError("elementwise operations need lists of numbers")
+        return layout.offsets, inner
+    if isinstance(layout, C.IndexedArray):
+        layout = layout.project()
+    if isinstance(layout, C.NumpyArray):
+        return None, layout
+    raise NotImplementedError(f"elementwise operation on {layout!r}")
+
+
+def _broadcast_ufunc(ufunc, inputs):
+    """Elementwise ufunc over lists that share offsets (the no-kernel case of
+    _broadcasting.py:363-396), or over lists and scalars."""
+    backend = None
+    structure = None
+    args = []
+    for x in inputs:
+        if isinstance(x, Array):
+            backend = x.backend
+            offsets, leaf = _leaf(x.layout)
+            if offsets is not None:
+                if structure is None:
+                    structure = offsets
+                elif structure is not offsets:
+                    nplike = backend.nplike
+                    a, b = nplike.to_numpy(structure), nplike.to_numpy(offsets)
+                    if a.shape != b.shape or not (a == b).all():
+                        raise ValueError("cannot broadcast nested list")
+            args.append(leaf.data)
+        else:
+            args.append(x)
+    data = backend.nplike.apply_ufunc(ufunc, *args)
+    leaf = C.NumpyArray(data, backend)
+    if structure is None:
+        return Array(leaf)
+    return Array(C.ListOffsetArray(structure, leaf, backend))
+
+
+def _getitem_jagged(layout, item):
+    """layout[item] for a jagged boolean or integer item with one list level
+    (Content._getitem, content.py:561-656; _slicing.py:476-530)."""
+    if not layout.is_list:
+        raise NotImplementedError("jagged getitem needs a list array")
+    if isinstance(item, C.ListArray):
+        item = item.to_ListOffsetArray64()
+    if not isinstance(item, C.ListOffsetArray):
+        raise NotImplementedError("jagged getitem needs a list index")
+    backend = layout.backend
+    nplike = backend.nplike
+    content = item.content
+    if isinstance(content, C.IndexedArray):
+        content = content.project()
+    offsets = item.offsets
+    if nplike.dtype(offsets) != np.dtype(np.int64):
+        offsets = nplike.asarray(offsets, np.int64)
+    if content.dtype == np.dtype(np.bool_):
+        index, nextoffsets = nplike.bool_to_jagged_index(offsets, content.data)
+        item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
+    else:
+        nextoffsets, trimmed = item._zero_based()
+        index = trimmed.data
+        if nplike.dtype(index) != np.dtype(np.int64):
+            index = nplike.asarray(index, np.int64)
+        item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
+    target = layout.to_ListArray() if isinstance(layout, C.ListOffsetArray) else layout
+    return target._getitem_jagged(item)
+
+
+# ---- reductions -------------------------------------------------------------------
+def _check_axis(axis):
+    if axis not in (-1, 1):
+        raise NotImplementedError("the hip jagged-array path reduces at axis=-1 (lists of numbers)")
+
+
+def _reduce(array, reducer, keepdims, mask_identity, axis):
+    _check_axis(axis)
+    return Array(C.reduce(array.layout, reducer, mask_identity, keepdims))
+
+
+def sum(array, axis=-1, keepdims=False, mask_identity=False):
+    return _reduce(array, R.Sum(), keepdims, mask_identity, axis)
+
+
+def max(array, axis=-1, keepdims=False, initial=None, mask_identity=True):
+    return _reduce(array, R.Max(initial), keepdims, mask_identity, axis)
+
+
+def min(array, axis=-1, keepdims=False, initial=None, mask_identity=True):
+    return _reduce(array, R.Min(initial), keepdims, mask_identity, axis)
+
+
+def argmax(array, axis=-1, keepdims=False, mask_identity=True):
+    return _reduce(array, R.ArgMax(), keepdims, mask_identity, axis)
+
+
+def argmin(array, axis=-1, keepdims=False, mask_identity=True):
+    return _reduce(array, R.ArgMin(), keepdims, mask_identity, axis)
+
+
+def count(array, axis=-1, keepdims=False, mask_identity=False):
+    return _reduce(array, R.Count(), keepdims, mask_identity, axis)
+
+
+def num(array, axis=1):
+    """List lengths (ak_num.py:133-142: stops - starts)."""
+    if axis not in (1, -1):
+        raise NotImplementedError("ak.num at axis=1 only")
+    layout = array.layout
+    nplike = layout.nplike
+    if isinstance(layout, C.ListOffsetArray):
+        starts, stops = layout.starts, layout.stops
+    elif isinstance(layout, C.ListArray):
+        starts, stops = layout.starts, layout.stops
+    else:
+        raise NotImplementedError("ak.num of a non-list array")
+    if nplike.dtype(starts) != np.dtype(np.int64):
+        starts, stops = nplike.asarray(starts, np.int64), nplike.asarray(stops, np.int64)
+    return Array(C.NumpyArray(nplike.subtract(stops, starts), layout.backend))
+
+
+def local_index(array, axis=-1):
+    _check_axis(axis)
+    layout = array.layout
+    if isinstance(layout, C.ListArray):
+        layout = layout.to_ListOffsetArray64()
+    return Array(layout._local_index())
+
+
+def combinations(array, n, replacement=False, axis=1, fields=None):
+    if axis not in (1, -1):
+        raise NotImplementedError("ak.combinations at axis=1 only")
+    layout = array.layout
+    if isinstance(layout, C.ListArray):
+        layout = layout.to_ListOffsetArray64()
+    if fields is not None and len(fields) != n:
+        raise ValueError("if provided, the length of 'fields' must be 'n'")
+    return Array(layout._combinations(n, replacement, fields))
+
+
+def argsort(array, axis=-1, ascending=True, stable=True):
+    _check_axis(axis)
+    layout = array.layout
+    if isinstance(layout, C.ListArray):
+        layout = layout.to_ListOffsetArray64()
+    return Array(layout._argsort_local(ascending, stable))
+
+
+def sort(array, axis=-1, ascending=True, stable=True):
+    _check_axis(axis)
+    layout = array.layout
+    if isinstance(layout, C.ListArray):
+        layout = layout.to_ListOffsetArray64()
+    return Array(layout._sort_local(ascending, stable))
+
+
+# ---- construction and backends ------------------------------------------------------
+def from_offsets(offsets, content, backend):
+    """Array of lists from host buffers: int64 offsets and either a 1-D numpy
+    array or a dict of equally long 1-D numpy arrays (a record)."""
+    nplike = backend.nplike
+    off = nplike.asarray(np.asarray(offsets, dtype=np.int64))
+    if isinstance(content, dict):
+        fields = list(content)
+        cols = [C.NumpyArray(nplike.asarray(np.ascontiguousarray(content[f])), backend) for f in fields]
+        length = len(next(iter(content.values()))) if content else 0
+        inner = C.RecordArray(cols, fields, length, backend)
+    else:
+        inner = C.NumpyArray(nplike.asarray(np.ascontiguousarray(content)), backend)
+    return Array(C.ListOffsetArray(off, inner, backend))
+
+
+def from_list(lists, backend, dtype=None):
+    """A list of lists of numbers (or of dicts of numbers) as an Array."""
+    counts = [len(x) for x in lists]
+    offsets = np.zeros(len(lists) + 1, dtype=np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    flat = [v for x in lists for v in x]
+    if flat and isinstance(flat[0], dict):
+        keys = list(flat[0])
+        content = {k: np.asarray([r[k] for r in flat], dtype=dtype) for k in keys}
+    else:
+        content = np.asarray(flat, dtype=dtype if dtype is not None else (np.float64 if not flat else None))
+    return from_offsets(offsets, content, backend)
+
+
+def to_backend(array, backend):
+    """Copy every buffer of the layout to `backend` (ak_to_backend.py:65-69)."""
+    if isinstance(backend, str):
+        if backend != "hip":
+            raise ValueError(f"unknown backend {backend!r}")
+        from awkward_amd.backend import hip_backend
+
+        backend = hip_backend()
+    return Array(_convert(array.layout, backend))
+
+
+def _convert(layout, backend):
+    src = layout.nplike
+    dst = backend.nplike
+
+    def move(x):
+        return dst.asarray(src.to_numpy(x))
+
+    if isinstance(layout, C.NumpyArray):
+        return C.NumpyArray(move(layout.data), backend)
+    if isinstance(layout, C.ListOffsetArray):
+        return C.ListOffsetArray(move(layout.offsets), _convert(layout.content, backend), backend)
+    if isinstance(layout, C.ListArray):
+        return C.ListArray(move(layout.starts), move(layout.stops), _convert(layout.content, backend), backend)
+    if isinstance(layout, C.RecordArray):
+        return C.RecordArray([_convert(c, backend) for c in layout.contents], layout.fields, layout.length, backend)
+    if isinstance(layout, C.IndexedArray):
+        return C.IndexedArray(move(layout.index), _convert(layout.content, backend), backend)
+    if isinstance(layout, C.ByteMaskedArray):
+        return C.ByteMaskedArray(move(layout.mask), _convert(layout.content, backend), layout.valid_when, backend)
+    raise NotImplementedError(type(layout).__name__)
+
+
+builtins_sum = builtins.sum

@@ -1,0 +1,136 @@
+"""Reducers: kernel choice and dtype rules of the reference's
+``src/awkward/_reducers.py`` for the axis=-1 path.
+
+* ``Sum`` (:332-455): bool, int8/16/32 accumulate in int64 and unsigned
+  narrower types in uint64 (``_promote_integer_rank`` :73-82 with the
+  Linux/numpy-2 answer ``_use32 = False``); float32 stays float32.
+* ``Max`` / ``Min`` (:861-1103): identity ``-inf`` / ``+inf`` or the integer
+  type's min / max unless ``initial`` is given; result keeps the dtype.
+* ``ArgMax`` / ``ArgMin`` (:132-251): int64 global indices, then
+  ``apply_positional_corrections`` (:85-129) subtracts the list starts; bool
+  data is viewed as int8.
+* ``Count`` (:254-279).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _promote_integer_rank(dtype: np.dtype) -> np.dtype:
+    if dtype in (np.dtype(np.bool_), np.dtype(np.int8), np.dtype(np.int16), np.dtype(np.int32)):
+        return np.dtype(np.int64)
+    if dtype in (np.dtype(np.uint8), np.dtype(np.uint16), np.dtype(np.uint32)):
+        return np.dtype(np.uint64)
+    return dtype
+
+
+class Reducer:
+    name = "reducer"
+    needs_position = False
+
+
+class Sum(Reducer):
+    name = "sum"
+
+    def apply(self, array, offsets, starts, shifts, outlength):
+        nplike = array.nplike
+        dtype = array.dtype
+        if dtype == np.dtype(np.bool_):
+            result = nplike.empty(outlength, np.int64)
+            array._check(
+                array.backend["awkward_reduce_sum_int64_bool_64", np.int64, np.bool_, np.int64](
+                    result, array.data, offsets, outlength
+                )
+            )
+            return type(array)(result, array.backend)
+        out_dtype = _promote_integer_rank(dtype)
+        result = nplike.empty(outlength, out_dtype)
+        array._check(
+            array.backend["awkward_reduce_sum", out_dtype, dtype, np.int64](result, array.data, offsets, outlength)
+        )
+        return type(array)(result, array.backend)
+
+
+class _Extremum(Reducer):
+    kernel = ""
+    is_max = True
+
+    def __init__(self, initial=None):
+        self.initial = initial
+
+    def identity_for(self, dtype: np.dtype):
+        if self.initial is not None:
+            return self.initial
+        if dtype.kind in "iu":
+            info = np.iinfo(dtype)
+            return int(info.min if self.is_max else info.max)
+        return -np.inf if self.is_max else np.inf
+
+    def apply(self, array, offsets, starts, shifts, outlength):
+        dtype = array.dtype
+        if dtype == np.dtype(np.bool_):
+            raise NotImplementedError(f"ak.{self.name} of booleans on the hip backend")
+        result = array.nplike.empty(outlength, dtype)
+        identity = self.identity_for(dtype)
+        if dtype.kind == "f":
+            identity = float(identity)
+        array._check(
+            array.backend[self.kernel, dtype, dtype, np.int64](result, array.data, offsets, outlength, identity)
+        )
+        return type(array)(result, array.backend)
+
+
+class Max(_Extremum):
+    name = "max"
+    kernel = "awkward_reduce_max"
+    is_max = True
+
+
+class Min(_Extremum):
+    name = "min"
+    kernel = "awkward_reduce_min"
+    is_max = False
+
+
+class _Arg(Reducer):
+    kernel = ""
+    needs_position = True
+
+    def apply(self, array, offsets, starts, shifts, outlength):
+        nplike = array.nplike
+        data = array.data
+        dtype = array.dtype
+        if dtype == np.dtype(np.bool_):
+            data = nplike.view(data, np.int8)
+            dtype = np.dtype(np.int8)
+        result = nplike.empty(outlength, np.int64)
+        array._check(
+            array.backend[self.kernel, np.int64, dtype, np.int64, np.int64](result, data, offsets, starts, outlength)
+        )
+        # apply_positional_corrections (shifts is None on this path)
+        array._check(
+            array.backend["awkward_NumpyArray_reduce_adjust_starts_64", np.int64, np.int64, np.int64](
+                result, outlength, offsets, starts
+            )
+        )
+        return type(array)(result, array.backend)
+
+
+class ArgMax(_Arg):
+    name = "argmax"
+    kernel = "awkward_reduce_argmax"
+
+
+class ArgMin(_Arg):
+    name = "argmin"
+    kernel = "awkward_reduce_argmin"
+
+
+class Count(Reducer):
+    name = "count"
+
+    def apply(self, array, offsets, starts, shifts, outlength):
+        result = array.nplike.empty(outlength, np.int64)
+        array._check(array.backend["awkward_reduce_count_64", np.int64, np.int64](result, offsets, outlength))
+        return type(array)(result, array.backend)

@@ -193,3 +193,39 @@ def check_case(fn, case: Case, memory):
         else:
             for g, e in zip(got, case.expected(argname)):
                 compare(case.name, argname, g, e)
+
+
+class DeviceMemory:
+    """Buffers copied to torch tensors on cuda:0; pointers are device addresses."""
+
+    def prepare(self, case: Case):
+        import torch
+
+        from awkward_amd._kernels import PointerArray
+
+        keep, call = [], []
+        for kind, v in case.args:
+            if kind == "scalar":
+                call.append(v)
+            elif kind == "ptr":
+                t = torch.from_numpy(np.ascontiguousarray(v)).to("cuda")
+                keep.append(t)
+                call.append(t.data_ptr())
+            else:
+                ts = [torch.from_numpy(np.ascontiguousarray(b)).to("cuda") for b in v]
+                table = PointerArray(ts)
+                keep.append(table)
+                call.append(table.data_ptr())
+        return keep, call
+
+    def fetch(self, case: Case, keep, index):
+        import torch
+
+        torch.cuda.synchronize()
+        kind, _ = case.args[index]
+        # keep holds one entry per non-scalar argument, in order
+        pos = sum(1 for k, _ in case.args[:index] if k != "scalar")
+        item = keep[pos]
+        if kind == "ptr":
+            return item.cpu().numpy()
+        return [t.cpu().numpy() for t in item.tensors]

@@ -1,0 +1,44 @@
+"""The C-ABI library loads without a GPU and exports every declared symbol."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from awkward_amd._kernel_signatures import SPECIALIZATIONS
+from awkward_amd._lib import LIBPATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    with open(os.path.join(ROOT, "include", "awkward_hip.h")) as f:
+        text = f.read()
+    return re.findall(r"^\S.*?\b(awkward_\w+)\(", text, flags=re.M)
+
+
+def test_header_declares_every_specialization():
+    names = set(declared())
+    missing = [n for n in SPECIALIZATIONS if n not in names]
+    assert not missing, missing
+    assert len(SPECIALIZATIONS) == 163
+
+
+@pytest.mark.skipif(not os.path.exists(LIBPATH), reason="libawkward_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIBPATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_product_does_not_import_oracle():
+    # the hip package never loads the oracle (test infrastructure only)
+    pkg = os.path.join(ROOT, "awkward_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".hip", ".h")):
+                with open(os.path.join(dirpath, fn)) as f:
+                    text = f.read()
+                assert "import oracle" not in text and "from oracle" not in text, fn
+                assert "liboracle" not in text, fn
