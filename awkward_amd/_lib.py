@@ -19,6 +19,17 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+def _warn_if_stale() -> None:
+    import glob
+    import sys
+
+    built = os.path.getmtime(LIBPATH)
+    sources = glob.glob(os.path.join(HERE, "csrc", "*"))
+    if any(os.path.getmtime(s) > built + 1 for s in sources):
+        print(f"warning: {LIBPATH} is older than its sources; rebuild with "
+              "`python -m awkward_amd.build`", file=sys.stderr)
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -27,6 +38,7 @@ def lib() -> ctypes.CDLL:
                 f"{LIBPATH} is not built; run `python -m awkward_amd.build` "
                 "(the hip backend has no CPU fallback)"
             )
+        _warn_if_stale()
         handle = ctypes.CDLL(LIBPATH)
         handle.awkward_hip_set_stream.argtypes = [ctypes.c_void_p]
         handle.awkward_hip_set_stream.restype = None

@@ -86,12 +86,17 @@ class KernelBackend:
 
 
 class HipBackend(KernelBackend):
+    """``fused=True`` (default): the layout code calls one tuned kernel per
+    reduction (see awkward_amd.reducers); ``fused=False`` issues the
+    reference's exact kernel sequence through the registry."""
+
     name = "hip"
 
-    def __init__(self, device=None):
+    def __init__(self, device=None, fused=True):
         from awkward_amd.nplike import TorchHip
 
         super().__init__(TorchHip(device))
+        self.fused = fused
 
     def _make_kernel(self, name: str):
         from awkward_amd._kernels import HipKernel

@@ -74,14 +74,15 @@ class NumpyArray(Content):
         return self
 
     def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims):
-        out = reducer.apply(self, offsets, starts, None, outlength)
-        if mask:
+        out, outmask = reducer.apply(self, offsets, starts, None, outlength, mask)
+        if mask and outmask is None:
             outmask = self.nplike.empty(outlength, np.int8)
             self._check(
                 self._kernel("awkward_NumpyArray_reduce_mask_ByteMaskedArray_64", np.int8, np.int64)(
                     outmask, offsets, outlength
                 )
             )
+        if mask:
             out = ByteMaskedArray(outmask, out, False, self.backend)
         if keepdims:
             out = RegularArray(out, 1, outlength, self.backend)

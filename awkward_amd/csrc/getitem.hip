@@ -87,6 +87,9 @@ __device__ inline int64_t range_count(int64_t length, int64_t start, int64_t sto
     if (b > length - 1) b = length - 1;
     if (b > a) b = a;
     *first = a;
+    // step == 0 takes this branch too (step > 0 is false), where the CPU
+    // loop visits nothing when a <= b and never ends otherwise: count 0.
+    if (step == 0) return 0;
     const int64_t s = -step;
     return (a > b) ? 1 + (a - b - 1) / s : 0;
   }
@@ -388,7 +391,6 @@ Error jagged_descend(int64_t* tooffsets, const int64_t* slicestarts, const int64
 template <typename C>
 Error next_range(C* tooffsets, int64_t* tocarry, const C* fromstarts, const C* fromstops, int64_t lenstarts,
                  int64_t start, int64_t stop, int64_t step) {
-  if (step == 0) return failure("slice step cannot be zero", kSliceNone, kSliceNone, kFile);
   if (lenstarts <= 0) {
     fill_c<C><<<1, 1, 0, stream()>>>(tooffsets, (C)0);
     AKHIP_LAUNCH_CHECK("fill_c");
@@ -401,7 +403,6 @@ Error next_range(C* tooffsets, int64_t* tocarry, const C* fromstarts, const C* f
 template <typename C>
 Error next_range_carrylength(int64_t* carrylength, const C* fromstarts, const C* fromstops, int64_t lenstarts,
                              int64_t start, int64_t stop, int64_t step) {
-  if (step == 0) return failure("slice step cannot be zero", kSliceNone, kSliceNone, kFile);
   return sum_to_scalar(RangeCountBody<C>{fromstarts, fromstops, start, stop, step}, lenstarts, carrylength);
 }
 

@@ -2,7 +2,8 @@
 //
 // One kernel template serves awkward_reduce_{sum,max,min,argmax,argmin} and
 // the bool sums (awkward-cpp/src/cpu-kernels/awkward_reduce_*.cpp).  A
-// workgroup of BLOCK threads owns BLOCK consecutive lists:
+// workgroup of BLOCK threads owns BLOCK * LPT consecutive lists (thread t the
+// lists t + j * BLOCK, so loads of offsets and stores of results coalesce):
 //
 //   * light tile (the lists' elements fit in CAP_BYTES of LDS): the span is
 //     staged with coalesced loads, then each thread reduces its own list
@@ -203,7 +204,7 @@ struct PlainEpilogue {
   __device__ Out apply(Out v, int64_t, int64_t, int64_t) const { return v; }
 };
 
-template <class Op, int BLOCK, int CAP_BYTES, class Epi = PlainEpilogue>
+template <class Op, int BLOCK, int LPT, int CAP_BYTES, class Epi = PlainEpilogue>
 __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi, typename Op::Out* __restrict__ toptr,
                                                                  const typename Op::In* __restrict__ fromptr,
                                                                  const int64_t* __restrict__ offsets,
@@ -213,24 +214,37 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
   using Part = typename Op::Part;
   constexpr int CAP = CAP_BYTES / (int)sizeof(In);
   constexpr int NW = BLOCK / kWave;
+  constexpr int L = BLOCK * LPT;  // lists per workgroup; thread t owns lists t + j * BLOCK
   __shared__ In s_data[CAP];
-  __shared__ int64_t s_off[BLOCK + 1];
-  __shared__ uint64_t s_long[NW];
+  __shared__ int64_t s_off[L + 1];
+  __shared__ uint64_t s_long[LPT][NW];
   __shared__ Part s_part[NW];
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-  const int64_t L0 = (int64_t)blockIdx.x * BLOCK;
-  const int nl = (int)((outlength - L0) < BLOCK ? (outlength - L0) : BLOCK);
+  const int64_t L0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)((outlength - L0) < L ? (outlength - L0) : L);
 
-  for (int t = tid; t <= nl; t += BLOCK) s_off[t] = offsets[L0 + t];
+  {
+    int64_t r[LPT + 1];
+#pragma unroll
+    for (int j = 0; j <= LPT; j++) {
+      const int t = tid + j * BLOCK;
+      if (t <= nl) r[j] = offsets[L0 + t];
+    }
+#pragma unroll
+    for (int j = 0; j <= LPT; j++) {
+      const int t = tid + j * BLOCK;
+      if (t <= nl) s_off[t] = r[j];
+    }
+  }
   __syncthreads();
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
 
   if (span <= CAP) {
-    // light tile: stage, then one serial loop per thread
+    // light tile: stage the span with coalesced loads, then serial loops
     constexpr int U = 8;
     for (int64_t b = 0; b < span; b += (int64_t)BLOCK * U) {
       In r[U];
@@ -246,60 +260,132 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
       }
     }
     __syncthreads();
-    if (tid < nl) {
-      const int64_t gb = s_off[tid], ge = s_off[tid + 1];
-      const Out v = op.sequential(s_data, gb - E0, ge - E0, gb);
-      toptr[L0 + tid] = epi.apply(v, L0 + tid, gb, ge);
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+      const int t = tid + j * BLOCK;
+      if (t < nl) {
+        const int64_t gb = s_off[t], ge = s_off[t + 1];
+        const Out v = op.sequential(s_data, gb - E0, ge - E0, gb);
+        toptr[L0 + t] = epi.apply(v, L0 + t, gb, ge);
+      }
     }
     return;
   }
 
   // heavy tile
-  int64_t gb = 0, ge = 0;
-  if (tid < nl) {
-    gb = s_off[tid];
-    ge = s_off[tid + 1];
-    if (ge - gb <= kShortLen) toptr[L0 + tid] = epi.apply(op.sequential(fromptr, gb, ge, gb), L0 + tid, gb, ge);
-  }
-  const int64_t len = ge - gb;
-  // medium lists: the owning wave reduces each one cooperatively
-  uint64_t medium = __ballot(tid < nl && len > kShortLen && len <= kWaveLen);
-  while (medium) {
-    const int b = __builtin_ctzll(medium);
-    medium &= medium - 1;
-    const int owner = wave * kWave + b;
-    const int64_t lb = s_off[owner], le = s_off[owner + 1];
-    Part p = op.init();
-    for (int64_t i = lb + lane; i < le; i += kWave) p = op.combine(p, op.elem(fromptr[i], i));
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
-    if (lane == 0) toptr[L0 + owner] = epi.apply(op.finalize(p, fromptr, lb, le), L0 + owner, lb, le);
-  }
-  // long lists: the whole workgroup
-  const uint64_t lng = __ballot(tid < nl && len > kWaveLen);
-  if (lane == 0) s_long[wave] = lng;
-  __syncthreads();
-  for (int w = 0; w < NW; w++) {
-    uint64_t m = s_long[w];
-    while (m) {
-      const int b = __builtin_ctzll(m);
-      m &= m - 1;
-      const int owner = w * kWave + b;
+  for (int j = 0; j < LPT; j++) {
+    const int t = tid + j * BLOCK;
+    int64_t gb = 0, ge = 0;
+    if (t < nl) {
+      gb = s_off[t];
+      ge = s_off[t + 1];
+      if (ge - gb <= kShortLen) toptr[L0 + t] = epi.apply(op.sequential(fromptr, gb, ge, gb), L0 + t, gb, ge);
+    }
+    const int64_t len = ge - gb;
+    // medium lists: the owning wave reduces each one cooperatively
+    uint64_t medium = __ballot(t < nl && len > kShortLen && len <= kWaveLen);
+    while (medium) {
+      const int b = __builtin_ctzll(medium);
+      medium &= medium - 1;
+      const int owner = j * BLOCK + wave * kWave + b;
       const int64_t lb = s_off[owner], le = s_off[owner + 1];
       Part p = op.init();
-#pragma unroll 4
-      for (int64_t i = lb + tid; i < le; i += BLOCK) p = op.combine(p, op.elem(fromptr[i], i));
+      for (int64_t i = lb + lane; i < le; i += kWave) p = op.combine(p, op.elem(fromptr[i], i));
 #pragma unroll
       for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
-      if (lane == 0) s_part[wave] = p;
-      __syncthreads();
-      if (tid == 0) {
-        Part q = s_part[0];
-        for (int k = 1; k < NW; k++) q = op.combine(q, s_part[k]);
-        toptr[L0 + owner] = epi.apply(op.finalize(q, fromptr, lb, le), L0 + owner, lb, le);
-      }
-      __syncthreads();
+      if (lane == 0) toptr[L0 + owner] = epi.apply(op.finalize(p, fromptr, lb, le), L0 + owner, lb, le);
     }
+    const uint64_t lng = __ballot(t < nl && len > kWaveLen);
+    if (lane == 0) s_long[j][wave] = lng;
+  }
+  __syncthreads();
+  // long lists: the whole workgroup
+  for (int j = 0; j < LPT; j++) {
+    for (int w = 0; w < NW; w++) {
+      uint64_t m = s_long[j][w];
+      while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        const int owner = j * BLOCK + w * kWave + b;
+        const int64_t lb = s_off[owner], le = s_off[owner + 1];
+        Part p = op.init();
+#pragma unroll 4
+        for (int64_t i = lb + tid; i < le; i += BLOCK) p = op.combine(p, op.elem(fromptr[i], i));
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
+        if (lane == 0) s_part[wave] = p;
+        __syncthreads();
+        if (tid == 0) {
+          Part q = s_part[0];
+          for (int k = 1; k < NW; k++) q = op.combine(q, s_part[k]);
+          toptr[L0 + owner] = epi.apply(op.finalize(q, fromptr, lb, le), L0 + owner, lb, le);
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// Fused epilogue of the host layer's local (axis=-1) arg-reductions: the
+// global index becomes local (awkward_NumpyArray_reduce_adjust_starts_64 with
+// starts = offsets[:-1]) and the empty-list mask is written in the same pass
+// (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64).
+struct LocalArgEpilogue {
+  int8_t* mask;  // may be null
+  __device__ int64_t apply(int64_t v, int64_t bin, int64_t gb, int64_t ge) const {
+    if (mask) mask[bin] = (ge - gb > 0) ? 0 : 1;
+    return v >= 0 ? v - gb : v;
+  }
+};
+
+// Fused mask for value reductions (max/min with mask_identity=True).
+struct MaskEpilogue {
+  int8_t* mask;
+  template <typename Out>
+  __device__ Out apply(Out v, int64_t bin, int64_t gb, int64_t ge) const {
+    if (mask) mask[bin] = (ge - gb > 0) ? 0 : 1;
+    return v;
+  }
+};
+
+
+template <class Op, int BLOCK, int LPT, int CAP, class Epi>
+inline Error launch_reduce_v(const Op& op, const Epi& epi, typename Op::Out* toptr, const typename Op::In* fromptr,
+                             const int64_t* offsets, int64_t outlength) {
+  if (outlength <= 0) return success();
+  const unsigned grid = (unsigned)ceil_div(outlength, (int64_t)BLOCK * LPT);
+  segmented_reduce_kernel<Op, BLOCK, LPT, CAP, Epi>
+      <<<grid, BLOCK, 0, stream()>>>(op, epi, toptr, fromptr, offsets, outlength);
+  AKHIP_LAUNCH_CHECK("segmented_reduce_kernel");
+  return success();
+}
+
+// Tile variants (threads, lists per thread, LDS staging bytes).  Variant 0
+// is the fixed default of the plain entry points (tiles sized for lists of
+// ~4-30 elements); the host layer picks others from the average list length
+// (elements / lists, known from the array shapes without a device sync), and
+// the autotuner and the learned dispatcher search all of them.
+//   0: 256 x 1, 32 KiB    1: 256 x 4, 8 KiB     2: 256 x 2, 16 KiB
+//   3: 128 x 1, 32 KiB    4: 256 x 1, 64 KiB    5: 256 x 8, 8 KiB
+constexpr int kVariants = 6;
+
+template <class Op>
+inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
+                           const int64_t* offsets, int64_t outlength) {
+  return launch_reduce_v<Op, 256, 1, 32768>(op, PlainEpilogue{}, toptr, fromptr, offsets, outlength);
+}
+
+template <class Op, class Epi>
+inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
+                           const int64_t* offsets, int64_t outlength, int variant, Epi epi) {
+  switch (variant) {
+    case 1: return launch_reduce_v<Op, 256, 4, 8192>(op, epi, toptr, fromptr, offsets, outlength);
+    case 2: return launch_reduce_v<Op, 256, 2, 16384>(op, epi, toptr, fromptr, offsets, outlength);
+    case 3: return launch_reduce_v<Op, 128, 1, 32768>(op, epi, toptr, fromptr, offsets, outlength);
+    case 4: return launch_reduce_v<Op, 256, 1, 65536>(op, epi, toptr, fromptr, offsets, outlength);
+    case 5: return launch_reduce_v<Op, 256, 8, 8192>(op, epi, toptr, fromptr, offsets, outlength);
+    default: return launch_reduce_v<Op, 256, 1, 32768>(op, epi, toptr, fromptr, offsets, outlength);
   }
 }
 

@@ -14,8 +14,12 @@
 //   * Body::emit(i, exclusive_prefix, count) writes the outputs.
 //
 // Status words pack a 2-bit flag with a 62-bit non-negative value, so one
-// 64-bit atomic carries both.  The look-back spin is bounded: a tile that
-// waited ~2^26 polls records an error instead of hanging the device.
+// 64-bit atomic carries both and no other memory needs ordering: the loads
+// and stores are relaxed atomics at agent scope.  (Acquire/release at agent
+// scope would write back / invalidate the XCD's L2 on every poll, since L2
+// is not coherent across the 8 XCDs -- that costs more than the whole scan.)
+// The look-back spin is bounded: a tile that waited ~2^26 polls records an
+// error instead of hanging the device.
 #pragma once
 
 #include "common.h"
@@ -87,11 +91,11 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
     int64_t prefix = 0;
     if (tile == 0) {
       if (lane == 0)
-        __hip_atomic_store(&s.status[0], kScanPrefix | (uint64_t)block_total, __ATOMIC_RELEASE,
+        __hip_atomic_store(&s.status[0], kScanPrefix | (uint64_t)block_total, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     } else {
       if (lane == 0)
-        __hip_atomic_store(&s.status[tile], kScanAgg | (uint64_t)block_total, __ATOMIC_RELEASE,
+        __hip_atomic_store(&s.status[tile], kScanAgg | (uint64_t)block_total, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
       int64_t look = tile - 1;
       bool done = false;
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
         if (idx >= 0) {
           uint32_t spins = 0;
           do {
-            w = __hip_atomic_load(&s.status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            w = __hip_atomic_load(&s.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((w & kScanFlags) != 0) break;
             __builtin_amdgcn_s_sleep(1);
           } while (++spins < (1u << 26));
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
       }
       if (lane == 0)
         __hip_atomic_store(&s.status[tile], kScanPrefix | (uint64_t)(prefix + block_total),
-                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) s_prefix = prefix;
   }

@@ -1,0 +1,115 @@
+"""Tile-variant choice for the segmented reductions, and the calls of the
+library's tuned/fused entry points (``awkward_hip_reduce_*``).
+
+The variants (threads, lists per thread, LDS staging bytes) are the ones
+compiled into libawkward_hip.so (``csrc/reduce.h``, ``launch_reduce``).  The
+fixed heuristic below picks the variant with the most lists per workgroup
+whose expected staged span (lists x mean list length x element size) fills
+at most 85% of the tile's LDS; the mean list length is
+elements / lists, which the host knows from the array shapes without
+synchronising with the device.  ``awkward_amd.tuning`` replaces the
+heuristic by measured choices (the autotuner table, or the learned
+dispatcher).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+VARIANTS = [
+    (256, 1, 32768),
+    (256, 4, 8192),
+    (256, 2, 16384),
+    (128, 1, 32768),
+    (256, 1, 65536),
+    (256, 8, 8192),
+]
+
+#: optional override: callable(op, n_lists, n_elements, itemsize) -> variant
+chooser = None
+
+
+def heuristic_variant(n_lists: int, n_elements: int, itemsize: int) -> int:
+    mean = n_elements / max(n_lists, 1)
+    best, best_key = 0, None
+    for v, (block, lpt, cap) in enumerate(VARIANTS):
+        lists = block * lpt
+        fill = lists * mean * itemsize / cap
+        if fill > 0.85:
+            continue
+        key = (lists, -cap)
+        if best_key is None or key > best_key:
+            best, best_key = v, key
+    if best_key is None:  # long lists: the largest staging tile, heavy paths
+        best = 4
+    return best
+
+
+def choose_variant(op: str, n_lists: int, n_elements: int, itemsize: int) -> int:
+    if chooser is not None:
+        return int(chooser(op, n_lists, n_elements, itemsize))
+    return heuristic_variant(n_lists, n_elements, itemsize)
+
+
+_TYPE = {
+    "int8": ctypes.c_int8, "uint8": ctypes.c_uint8, "int16": ctypes.c_int16, "uint16": ctypes.c_uint16,
+    "int32": ctypes.c_int32, "uint32": ctypes.c_uint32, "int64": ctypes.c_int64, "uint64": ctypes.c_uint64,
+    "float32": ctypes.c_float, "float64": ctypes.c_double,
+}
+_fns = {}
+
+
+def _fn(name, argtypes):
+    f = _fns.get(name)
+    if f is None:
+        from awkward_amd._kernels import KernelError
+        from awkward_amd._lib import lib
+
+        f = getattr(lib(), name)
+        f.argtypes = argtypes
+        f.restype = KernelError
+        _fns[name] = f
+    return f
+
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+INT = ctypes.c_int
+
+
+def reduce_sum(out, data, offsets, n, variant):
+    from awkward_amd._kernels import device_address, set_stream
+
+    on = np.dtype(_np(out.dtype)).name
+    inn = np.dtype(_np(data.dtype)).name
+    f = _fn(f"awkward_hip_reduce_sum_{on}_{inn}", [P, P, P, I64, INT])
+    set_stream()
+    return f(device_address(out), device_address(data), device_address(offsets), n, variant)
+
+
+def reduce_extremum(kind, out, mask, data, offsets, n, identity, variant):
+    from awkward_amd._kernels import device_address, set_stream
+
+    tn = np.dtype(_np(data.dtype)).name
+    f = _fn(f"awkward_hip_reduce_{kind}_masked_{tn}", [P, P, P, P, I64, _TYPE[tn], INT])
+    set_stream()
+    return f(device_address(out), 0 if mask is None else device_address(mask), device_address(data),
+             device_address(offsets), n, identity, variant)
+
+
+def reduce_arg(kind, out, mask, data, offsets, n, variant):
+    from awkward_amd._kernels import device_address, set_stream
+
+    tn = np.dtype(_np(data.dtype)).name
+    f = _fn(f"awkward_hip_reduce_{kind}_local_{tn}", [P, P, P, P, I64, INT])
+    set_stream()
+    return f(device_address(out), 0 if mask is None else device_address(mask), device_address(data),
+             device_address(offsets), n, variant)
+
+
+def _np(dtype):
+    from awkward_amd.nplike import to_numpy_dtype
+
+    return to_numpy_dtype(dtype)

@@ -10,6 +10,14 @@
   ``apply_positional_corrections`` (:85-129) subtracts the list starts; bool
   data is viewed as int8.
 * ``Count`` (:254-279).
+
+``apply`` returns ``(result, mask)``.  On a backend with ``fused = True``
+(the hip backend) the result comes from one tuned kernel per reduction --
+with argmin/argmax's start correction and mask_identity's empty-list mask
+written in the same pass, and the tile variant chosen by
+``awkward_amd.dispatch`` -- instead of the reference's sequence of separate
+kernels (reduce, adjust_starts, reduce_mask), which other backends run.
+Both give the same bits.
 """
 
 from __future__ import annotations
@@ -30,26 +38,41 @@ class Reducer:
     needs_position = False
 
 
+def _fused(array):
+    return getattr(array.backend, "fused", False)
+
+
+def _variant(op, array, outlength):
+    from awkward_amd import dispatch
+
+    return dispatch.choose_variant(op, outlength, array.length, array.dtype.itemsize)
+
+
 class Sum(Reducer):
     name = "sum"
 
-    def apply(self, array, offsets, starts, shifts, outlength):
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
         nplike = array.nplike
         dtype = array.dtype
-        if dtype == np.dtype(np.bool_):
-            result = nplike.empty(outlength, np.int64)
+        out_dtype = np.dtype(np.int64) if dtype == np.dtype(np.bool_) else _promote_integer_rank(dtype)
+        result = nplike.empty(outlength, out_dtype)
+        if _fused(array):
+            from awkward_amd import dispatch
+
+            array._check(dispatch.reduce_sum(result, array.data, offsets, outlength,
+                                             _variant("sum", array, outlength)))
+        elif dtype == np.dtype(np.bool_):
             array._check(
                 array.backend["awkward_reduce_sum_int64_bool_64", np.int64, np.bool_, np.int64](
                     result, array.data, offsets, outlength
                 )
             )
-            return type(array)(result, array.backend)
-        out_dtype = _promote_integer_rank(dtype)
-        result = nplike.empty(outlength, out_dtype)
-        array._check(
-            array.backend["awkward_reduce_sum", out_dtype, dtype, np.int64](result, array.data, offsets, outlength)
-        )
-        return type(array)(result, array.backend)
+        else:
+            array._check(
+                array.backend["awkward_reduce_sum", out_dtype, dtype, np.int64](result, array.data, offsets,
+                                                                                outlength)
+            )
+        return type(array)(result, array.backend), None
 
 
 class _Extremum(Reducer):
@@ -67,7 +90,7 @@ class _Extremum(Reducer):
             return int(info.min if self.is_max else info.max)
         return -np.inf if self.is_max else np.inf
 
-    def apply(self, array, offsets, starts, shifts, outlength):
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
         dtype = array.dtype
         if dtype == np.dtype(np.bool_):
             raise NotImplementedError(f"ak.{self.name} of booleans on the hip backend")
@@ -75,10 +98,17 @@ class _Extremum(Reducer):
         identity = self.identity_for(dtype)
         if dtype.kind == "f":
             identity = float(identity)
+        if _fused(array):
+            from awkward_amd import dispatch
+
+            outmask = array.nplike.empty(outlength, np.int8) if mask else None
+            array._check(dispatch.reduce_extremum(self.name, result, outmask, array.data, offsets, outlength,
+                                                  identity, _variant(self.name, array, outlength)))
+            return type(array)(result, array.backend), outmask
         array._check(
             array.backend[self.kernel, dtype, dtype, np.int64](result, array.data, offsets, outlength, identity)
         )
-        return type(array)(result, array.backend)
+        return type(array)(result, array.backend), None
 
 
 class Max(_Extremum):
@@ -97,7 +127,7 @@ class _Arg(Reducer):
     kernel = ""
     needs_position = True
 
-    def apply(self, array, offsets, starts, shifts, outlength):
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
         nplike = array.nplike
         data = array.data
         dtype = array.dtype
@@ -105,6 +135,13 @@ class _Arg(Reducer):
             data = nplike.view(data, np.int8)
             dtype = np.dtype(np.int8)
         result = nplike.empty(outlength, np.int64)
+        if _fused(array):
+            from awkward_amd import dispatch
+
+            outmask = nplike.empty(outlength, np.int8) if mask else None
+            array._check(dispatch.reduce_arg(self.name, result, outmask, data, offsets, outlength,
+                                             _variant(self.name, array, outlength)))
+            return type(array)(result, array.backend), outmask
         array._check(
             array.backend[self.kernel, np.int64, dtype, np.int64, np.int64](result, data, offsets, starts, outlength)
         )
@@ -114,7 +151,7 @@ class _Arg(Reducer):
                 result, outlength, offsets, starts
             )
         )
-        return type(array)(result, array.backend)
+        return type(array)(result, array.backend), None
 
 
 class ArgMax(_Arg):
@@ -130,7 +167,7 @@ class ArgMin(_Arg):
 class Count(Reducer):
     name = "count"
 
-    def apply(self, array, offsets, starts, shifts, outlength):
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
         result = array.nplike.empty(outlength, np.int64)
         array._check(array.backend["awkward_reduce_count_64", np.int64, np.int64](result, offsets, outlength))
-        return type(array)(result, array.backend)
+        return type(array)(result, array.backend), None

@@ -220,10 +220,10 @@ def test_regular_next_range():
 def test_next_at_compact_nextcarry(c):
     rng = np.random.default_rng(19)
     n = 4000
-    counts = rng.integers(1, 9, n)
+    counts = rng.integers(2, 9, n)
     off = offsets_from_counts(counts)
     fs, fe = off[:-1].astype(c), off[1:].astype(c)
-    for at in (0, -1, 1):
+    for at in (0, -1, 1, -2):
         h, d, he, de = run_both(f"awkward_ListArray{P[c]}_getitem_next_at_64", [np.zeros(n, np.int64), fs, fe, n, at])
         assert he.str is None and de.str is None and same_bits(h[0], d[0])
     h, d, he, de = run_both(f"awkward_ListArray{P[c]}_getitem_next_at_64", [np.zeros(n, np.int64), fs, fe, n, 3])
@@ -315,3 +315,45 @@ def test_local_preparenext():
     x = rng.integers(0, 50, 10000).astype(np.int64)
     h, d, _, _ = run_both("awkward_ListOffsetArray_local_preparenext_64", [np.zeros(10000, np.int64), x, 10000])
     assert same_bits(h[0], d[0])
+
+
+@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("t", ["int32", "int64", "float32", "float64"])
+def test_tuned_variants(variant, t):
+    """The tuned/fused entry points, every tile variant, against the oracle's
+    plain kernels followed by adjust_starts / reduce_mask."""
+    import torch
+
+    from awkward_amd import dispatch
+
+    rng = np.random.default_rng(30 + variant)
+    for label, counts in [("sparse", np.clip(rng.poisson(0.4, 20000), 0, 20)),
+                          ("short", mixed_counts(rng, 6000, lam=10)),
+                          ("mixed", mixed_counts(rng, 3000, long_every=499, medium_every=7))]:
+        off = offsets_from_counts(counts)
+        n = len(counts)
+        x = values(rng, t, int(off[-1]), special=False)
+        d_off = torch.from_numpy(off).cuda()
+        d_x = torch.from_numpy(x).cuda()
+        # sum
+        out_t = {"int32": "int64", "int64": "int64", "float32": "float32", "float64": "float64"}[t]
+        h, _, _, _ = run_both(f"awkward_reduce_sum_{out_t}_{t}_64", [np.zeros(n, out_t), x, off, n])
+        d_out = torch.empty(n, dtype=getattr(torch, out_t), device="cuda")
+        assert dispatch.reduce_sum(d_out, d_x, d_off, n, variant).str is None
+        got = d_out.cpu().numpy()
+        if out_t.startswith("int") or label != "mixed":
+            assert same_bits(h[0], got), ("sum", label, variant)
+        # max with mask
+        ident = -np.inf if t.startswith("float") else int(np.iinfo(t).min)
+        h, _, _, _ = run_both(f"awkward_reduce_max_{t}_{t}_64", [np.zeros(n, t), x, off, n, ident])
+        hm, _, _, _ = run_both("awkward_NumpyArray_reduce_mask_ByteMaskedArray_64", [np.zeros(n, np.int8), off, n])
+        d_out = torch.empty(n, dtype=getattr(torch, t), device="cuda")
+        d_mask = torch.empty(n, dtype=torch.int8, device="cuda")
+        assert dispatch.reduce_extremum("max", d_out, d_mask, d_x, d_off, n, ident, variant).str is None
+        assert same_bits(h[0], d_out.cpu().numpy()) and same_bits(hm[0], d_mask.cpu().numpy()), ("max", label)
+        # argmin, local
+        h, _, _, _ = run_both(f"awkward_reduce_argmin_{t}_64", [np.zeros(n, np.int64), x, off, off[:-1].copy(), n])
+        want = np.where(h[0] >= 0, h[0] - off[:-1], h[0])
+        d_out = torch.empty(n, dtype=torch.int64, device="cuda")
+        assert dispatch.reduce_arg("argmin", d_out, d_mask, d_x, d_off, n, variant).str is None
+        assert same_bits(want, d_out.cpu().numpy()) and same_bits(hm[0], d_mask.cpu().numpy()), ("argmin", label)

@@ -10,12 +10,14 @@ from parity import same_bits
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def backends():
-    from awkward_amd.backend import hip_backend
+@pytest.fixture(scope="module", params=["fused", "plain"])
+def backends(request):
+    """The oracle backend and a hip backend: fused (one tuned kernel per
+    reduction) or plain (the reference's kernel sequence)."""
+    from awkward_amd.backend import HipBackend
     from oracle.backend import OracleBackend
 
-    return OracleBackend(), hip_backend()
+    return OracleBackend(), HipBackend(fused=request.param == "fused")
 
 
 def events(seed, n, lam, clip):
@@ -91,9 +93,7 @@ def test_config3_pipeline(backends, n):
     # cosh/cos come from different math libraries: tolerance on the values,
     # and the reductions are checked on the device's own expression values
     assert np.allclose(vc, vh, rtol=1e-5, atol=1e-3)
-    from oracle.backend import OracleBackend
-
-    ev_h = ak.Array(ak.highlevel.C.ListOffsetArray(off, ak.highlevel.C.NumpyArray(vh, cpu), cpu))
+    ev_h = ak.Array(ak.highlevel.C.ListOffsetArray(oh, ak.highlevel.C.NumpyArray(vh, cpu), cpu))
     for a, b in zip(flat(ak.max(ev_h, axis=-1)), flat(m_h)):
         assert same_bits(a, b)
     for a, b in zip(flat(ak.argmax(ev_h, axis=-1)), flat(a_h)):
