@@ -1,0 +1,179 @@
+"""Fused, synchronisation-free forms of the configs' op sequences, and
+hipGraph capture of them.
+
+``EventPipeline`` evaluates, for every event of a list-of-records array,
+
+    sel = ev[(term_1) & (term_2) & ...][:, start:stop:step]
+    ak.num(sel), ak.sum(sel[field]), ak.argmax(sel[field]), ak.max(sel[field])
+
+in one kernel (``awkward_hip_event_pipeline``) with the same results as the
+per-kernel path; ``pair_max`` evaluates ak.max / ak.argmax over
+``ak.combinations(ev, 2)`` of the pair expression
+``2*pt1*pt2*(cosh(eta1-eta2) - cos(phi1-phi2))`` without materialising the
+pairs (``awkward_hip_pair_max``).  Neither reads anything back to the host,
+so ``Graph`` can capture them (with torch.cuda.CUDAGraph, which is a hipGraph
+on ROCm) and replay the whole sequence with one launch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from awkward_amd import contents as C
+from awkward_amd._kernels import KernelError, device_address, set_stream
+from awkward_amd._lib import lib
+
+_OPS = {">": 0, ">=": 1, "<": 2, "<=": 3, "==": 4, "!=": 5}
+_TYPES = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.int8): 2, np.dtype(np.int32): 3,
+          np.dtype(np.int64): 4}
+_NONE = 2**63 - 1
+
+_fn_cache = {}
+
+
+def _fn(name, argtypes):
+    f = _fn_cache.get(name)
+    if f is None:
+        f = getattr(lib(), name)
+        f.argtypes = argtypes
+        f.restype = KernelError
+        _fn_cache[name] = f
+    return f
+
+
+def _check(err):
+    if err.str is not None:
+        raise RuntimeError(err.str.decode())
+
+
+def _records(array):
+    layout = array.layout
+    if not isinstance(layout, C.ListOffsetArray) or not isinstance(layout.content, C.RecordArray):
+        raise NotImplementedError("fused pipelines take a list array of records")
+    return layout.offsets, layout.content
+
+
+class EventPipeline:
+    """terms: [(field, op, value) or (field, "abs"+op, value)], combined with &."""
+
+    def __init__(self, events, terms, slice_=slice(None), reduce_field="pt",
+                 outputs=("num", "sum", "argmax", "max")):
+        offsets, record = _records(events)
+        self.offsets = offsets
+        self.n = record.backend.nplike.length(offsets) - 1
+        names = []
+        for f, _, _ in terms:
+            if f not in names:
+                names.append(f)
+        if reduce_field not in names:
+            names.append(reduce_field)
+        cols = [record._getitem_field(f).data for f in names]
+        self.cols = cols
+        nplike = record.backend.nplike
+        self.field_ptrs = (ctypes.c_void_p * len(cols))(*[c.data_ptr() for c in cols])
+        self.field_types = (ctypes.c_int32 * len(cols))(*[_TYPES[nplike.dtype(c)] for c in cols])
+        tf, to, ta, tv = [], [], [], []
+        for f, op, value in terms:
+            absolute = op.startswith("abs")
+            op = op[3:] if absolute else op
+            tf.append(names.index(f))
+            to.append(_OPS[op])
+            ta.append(1 if absolute else 0)
+            tv.append(float(value))
+        k = len(terms)
+        self.terms = ((ctypes.c_int32 * k)(*tf), (ctypes.c_int32 * k)(*to), (ctypes.c_int32 * k)(*ta),
+                      (ctypes.c_double * k)(*tv), k)
+        self.start = _NONE if slice_.start is None else slice_.start
+        self.stop = _NONE if slice_.stop is None else slice_.stop
+        self.step = 1 if slice_.step is None else slice_.step
+        self.red = names.index(reduce_field)
+        dev = offsets.device
+        spec = {"num": [("num", torch.int64)], "sum": [("sum", torch.float32)],
+                "argmax": [("argmax", torch.int64), ("argmax_mask", torch.int8)],
+                "max": [("max", torch.float32), ("max_mask", torch.int8)]}
+        self.out = {}
+        for name in outputs:
+            for key, dt in spec[name]:
+                self.out[key] = torch.empty(self.n, dtype=dt, device=dev)
+        P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        self._f = _fn("awkward_hip_event_pipeline",
+                      [P, I64, P, P, I32, P, P, P, P, I32, I64, I64, I64, I32, P, P, P, P, P, P])
+
+    def run(self):
+        set_stream()
+        tf, to, ta, tv, k = self.terms
+        o = self.out
+        _check(self._f(device_address(self.offsets), self.n, ctypes.addressof(self.field_ptrs),
+                       ctypes.addressof(self.field_types), len(self.cols), ctypes.addressof(tf),
+                       ctypes.addressof(to), ctypes.addressof(ta), ctypes.addressof(tv), k, self.start, self.stop,
+                       self.step, self.red, *[o[k].data_ptr() if k in o else 0 for k in
+                                              ("num", "sum", "argmax", "argmax_mask", "max", "max_mask")]))
+        return o
+
+    def algorithmic_bytes(self, n_particles: int) -> int:
+        """Compulsory traffic: offsets and the used fields read once, every
+        requested output written once."""
+        n = self.n
+        read = 8 * (n + 1) + sum(c.element_size() * n_particles for c in self.cols)
+        return read + sum(v.element_size() * n for v in self.out.values())
+
+
+class PairMax:
+    """ak.max / ak.argmax over ak.combinations(ev, 2) of the pair expression."""
+
+    def __init__(self, events, pt="pt", eta="eta", phi="phi"):
+        offsets, record = _records(events)
+        self.offsets = offsets
+        self.n = record.backend.nplike.length(offsets) - 1
+        self.cols = [record._getitem_field(f).data for f in (pt, eta, phi)]
+        dev = offsets.device
+        self.out = {
+            "max": torch.empty(self.n, dtype=torch.float32, device=dev),
+            "argmax": torch.empty(self.n, dtype=torch.int64, device=dev),
+            "mask": torch.empty(self.n, dtype=torch.int8, device=dev),
+        }
+        P, I64 = ctypes.c_void_p, ctypes.c_int64
+        self._f = _fn("awkward_hip_pair_max", [P, I64, P, P, P, P, P, P, ctypes.c_float])
+
+    def run(self):
+        set_stream()
+        o = self.out
+        _check(self._f(device_address(self.offsets), self.n, *[c.data_ptr() for c in self.cols], o["max"].data_ptr(),
+                       o["argmax"].data_ptr(), o["mask"].data_ptr(), float("-inf")))
+        return o
+
+
+def pair_expr(p0, p1):
+    """The pair expression over gathered legs (float32 device tensors)."""
+    out = torch.empty_like(p0[0])
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    f = _fn("awkward_hip_pair_expr", [P, P, P, P, P, P, P, I64])
+    set_stream()
+    _check(f(out.data_ptr(), *[t.data_ptr() for t in p0], *[t.data_ptr() for t in p1], out.shape[0]))
+    return out
+
+
+class Graph:
+    """Captures fn() (which must not synchronise or allocate through the
+    library) into a hipGraph and replays it."""
+
+    def __init__(self, fn, warmup=2):
+        self.fn = fn
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                fn()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.result = fn()
+        torch.cuda.synchronize()
+
+    def replay(self):
+        self.graph.replay()
+        return self.result

@@ -9,9 +9,11 @@ Workload (one "step"): the jagged event pipeline on 50M events of records
     ak.num(sel), ak.sum(sel.pt, axis=-1), ak.argmax(sel.pt, axis=-1)
 
 Inputs are resident in HBM before timing.  `value` is events/s of the whole
-pipeline.  `roofline` is the HBM roofline of the pipeline's dominant kernel
-(algorithmic bytes / its average duration, from HIP events recorded on the
-kernels' stream inside the timed steps).  `cpu_baseline` times the same
+pipeline in its fused form (one kernel, replayed from a hipGraph); `forms`
+also times the reference's per-kernel sequence ("plain") and the per-kernel
+path with tuned reductions.  `roofline` is the HBM roofline of the fused
+form's kernel (algorithmic bytes / its average duration, from HIP events
+recorded around each replay on the stream it runs on).  `cpu_baseline` times the same
 pipeline through the same host layer on the single-threaded CPU oracle over a
 bounded sample of the events.  `config1` reports the segmented reductions of
 the 1M-list Poisson(10) float64 layout, HBM-cold (a 1 GiB scrub before each
@@ -63,62 +65,6 @@ def pipeline(ak, ev):
     return ak.num(sel), ak.sum(sel.pt, axis=-1), ak.argmax(sel.pt, axis=-1)
 
 
-class KernelTimer:
-    """Wraps registry kernels of a backend to record HIP events around each
-    launch on the current stream (the stream the library enqueues on)."""
-
-    def __init__(self, backend, names):
-        import torch
-
-        self.torch = torch
-        self.backend = backend
-        self.names = set(names)
-        self.events = {}
-        self.enabled = False
-        self._orig = backend.__class__.__getitem__
-
-        timer = self
-
-        def getitem(be, key):
-            kernel = timer._orig(be, key)
-            fam = key[0]
-            if fam in timer.names and timer.enabled:
-                return _Timed(kernel, fam, timer)
-            return kernel
-
-        backend.__class__.__getitem__ = getitem
-
-    def restore(self):
-        self.backend.__class__.__getitem__ = self._orig
-
-    def record(self, fam):
-        torch = self.torch
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
-        self.events.setdefault(fam, []).append((start, stop))
-        return start, stop
-
-    def mean_ms(self, fam):
-        torch = self.torch
-        torch.cuda.synchronize()
-        ev = self.events.get(fam, [])
-        if not ev:
-            return None
-        return sum(a.elapsed_time(b) for a, b in ev) / len(ev)
-
-
-class _Timed:
-    def __init__(self, kernel, fam, timer):
-        self.kernel, self.fam, self.timer = kernel, fam, timer
-
-    def __call__(self, *args):
-        start, stop = self.timer.record(self.fam)
-        start.record()
-        out = self.kernel(*args)
-        stop.record()
-        return out
-
-
 def run_config1(ak, backend, torch, reps=5):
     """Segmented reductions on 1M Poisson(10) float64 lists, HBM-cold."""
     import numpy as np
@@ -145,16 +91,28 @@ def run_config1(ak, backend, torch, reps=5):
                    (out_i, dev_x, dev_off, starts, n), 8 * (n + 1) + 8 * N + 8 * n),
         "count": (("awkward_reduce_count_64", np.int64, np.int64), (out_i, dev_off, n), 8 * (n + 1) + 8 * n),
     }
-    result = {"lists": n, "elements": N, "dtype": "float64", "method": "HBM-cold: 1 GiB scrub before each launch"}
+    from awkward_amd import dispatch
+
+    v = dispatch.heuristic_variant(n, N, 8)
+    mask = torch.empty(n, dtype=torch.int8, device="cuda")
+    ops["sum_tuned"] = (lambda: dispatch.reduce_sum(out_f, dev_x, dev_off, n, v), None, 8 * (n + 1) + 8 * N + 8 * n)
+    ops["argmax_tuned"] = (lambda: dispatch.reduce_arg("argmax", out_i, mask, dev_x, dev_off, n, v), None,
+                           8 * (n + 1) + 8 * N + 9 * n)
+    result = {"lists": n, "elements": N, "dtype": "float64", "variant": v,
+              "method": "HBM-cold: 1 GiB scrub before each launch; *_tuned = fused entry point, heuristic tile"}
     for name, (key, args, nbytes) in ops.items():
-        kernel = backend[key]
-        kernel(*args)
+        if args is None:
+            call = key
+        else:
+            kern = backend[key]
+            call = (lambda k, a: (lambda: k(*a)))(kern, args)
+        call()
         times = []
         for _ in range(reps):
             scrub.add_(1.0)
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record()
-            kernel(*args)
+            call()
             e.record()
             torch.cuda.synchronize()
             times.append(s.elapsed_time(e))
@@ -190,6 +148,18 @@ def cpu_baseline(ak, off, fields, sample_events):
                       f"through the same host layer on the single-threaded C++ oracle"}
 
 
+def time_form(torch, fn, steps, warmup, events):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"ms_per_step": round(dt * 1e3, 4), "events_per_s": round(events / dt, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,47 +178,59 @@ def main():
     import torch
 
     import awkward_amd as ak
-    from awkward_amd.backend import hip_backend
+    from awkward_amd.backend import HipBackend
+    from awkward_amd.fused import EventPipeline, Graph
 
     log(f"generating {args.events} events")
     off, fields = make_events(args.events)
-    backend = hip_backend()
-    ev = ak.from_offsets(off, fields, backend)
+    plain_be = HipBackend(fused=False)
+    tuned_be = HipBackend(fused=True)
+    ev_plain = ak.from_offsets(off, fields, plain_be)
+    ev = ak.from_offsets(off, fields, tuned_be)
     torch.cuda.synchronize()
     N = int(off[-1])
     log(f"resident: {args.events} events, {N} particles")
 
-    for _ in range(args.warmup):
-        pipeline(ak, ev)
-    torch.cuda.synchronize()
+    # the three forms of the same pipeline
+    forms = {}
+    short = max(3, args.steps // 4)
+    forms["plain"] = time_form(torch, lambda: pipeline(ak, ev_plain), short, 1, args.events)
+    forms["plain"]["what"] = "the reference's kernel sequence, one registry kernel per step, host syncs for sizes"
+    log(f"plain: {forms['plain']}")
+    forms["tuned"] = time_form(torch, lambda: pipeline(ak, ev), short, 1, args.events)
+    forms["tuned"]["what"] = "per-kernel path with fused/tuned reduction kernels (variant by mean list length)"
+    log(f"tuned: {forms['tuned']}")
 
-    dominant = "awkward_reduce_sum"
-    timer = KernelTimer(backend, [dominant])
-    timer.enabled = True
+    terms = [("pt", ">", 25.0), ("eta", "abs<", 2.4)]
+    pipe = EventPipeline(ev, terms, slice(1, None), "pt", outputs=("num", "sum", "argmax"))
+    graph = Graph(pipe.run)
+    for _ in range(args.warmup):
+        graph.replay()
     torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    stops = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        outs = pipeline(ak, ev)
+    for k in range(args.steps):
+        starts[k].record()
+        graph.replay()
+        stops[k].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    timer.enabled = False
-    timer.restore()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = args.events / ((t1 - t0) / args.steps)
-    log(f"{ms_per_step:.3f} ms/step, {value:.4g} events/s")
+    kms = sum(a.elapsed_time(b) for a, b in zip(starts, stops)) / args.steps
+    forms["fused_graph"] = {"ms_per_step": round(ms_per_step, 4), "events_per_s": round(value, 1),
+                            "what": "awkward_hip_event_pipeline, one kernel, replayed from a hipGraph"}
+    log(f"fused_graph: {forms['fused_graph']}")
 
-    # dominant kernel roofline: segmented sum over the sliced pt (float32)
-    sel_n = args.events
-    K = int(ak.num(ev[(ev.pt > 25) & (abs(ev.eta) < 2.4)][:, 1:]).layout.data.sum().item())
-    alg_bytes = 8 * (sel_n + 1) + 4 * K + 4 * sel_n
-    kms = timer.mean_ms(dominant)
-    achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
-                "kernel": "segmented_reduce_kernel (awkward_reduce_sum_float32_float32_64)",
-                "kernel_ms": round(kms, 4) if kms else None, "alg_bytes": alg_bytes}
+    alg_bytes = pipe.algorithmic_bytes(N)
+    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "event_pipeline_kernel (awkward_hip_event_pipeline)", "kernel_ms": round(kms, 4),
+                "alg_bytes": alg_bytes}
 
-    config1 = None if args.no_config1 else run_config1(ak, backend, torch)
+    config1 = None if args.no_config1 else run_config1(ak, tuned_be, torch)
     cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
 
     line = {
@@ -266,7 +248,9 @@ def main():
         "data": "synthetic (seeded generator of the event-pipeline workload)",
         "config": {"workload": "jagged event pipeline: 50M events {pt,eta,phi f32; charge i8}, "
                                "Poisson(3) clipped [0,20]; mask -> jagged getitem -> [:,1:] -> num/sum/argmax",
-                   "events": args.events, "particles": N, "path": "plain (per-kernel launches)"},
+                   "events": args.events, "particles": N, "path": "fused_graph"},
+        "forms": forms,
+        "speedup_fused_vs_plain": round(forms["fused_graph"]["events_per_s"] / forms["plain"]["events_per_s"], 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config1": config1,
