@@ -135,40 +135,67 @@ struct PipelineOut {
   int8_t* maxmask;
 };
 
-constexpr int kBlock = 256;
-constexpr int kStageFloats = 2048;  // per staged field (32 KiB for four)
+// Staging plan: slot[k] is field k's row in the LDS stage (-1: read from
+// global memory); the stage holds nslots rows of cap floats.
+struct Stage {
+  int slot[kMaxFields];
+  int nslots;
+  int cap;
+};
 
+__device__ inline bool keep_particle(const Predicate& pred, const Fields& fields, const Stage& st,
+                                     const float* s_stage, bool staged, int64_t i, int64_t local) {
+  for (int t = 0; t < pred.nterms; t++) {
+    const Term& term = pred.term[t];
+    const int slot = st.slot[term.field];
+    bool ok;
+    if (staged && slot >= 0) {
+      float x = s_stage[slot * st.cap + local];
+      if (term.absolute) x = fabsf(x);
+      ok = compare<float>(x, term.op, (float)term.value);
+    } else {
+      ok = eval_term(term, fields, nullptr, i, local, false);
+    }
+    if (!ok) return false;
+  }
+  return true;
+}
 
-__global__ __launch_bounds__(kBlock) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
-                                                                Fields fields, Predicate pred, int64_t start,
-                                                                int64_t stop, int64_t step, int red,
-                                                                PipelineOut out) {
-  __shared__ float s_field[kMaxFields][kStageFloats];
-  __shared__ int64_t s_off[kBlock + 1];
+// BLOCK threads, one event each; the workgroup's particles are staged with
+// coalesced loads into a stage sized at launch (dynamic LDS) to keep many
+// workgroups -- and many loads -- in flight per CU.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                                                               Fields fields, Predicate pred, Stage st,
+                                                               int64_t start, int64_t stop, int64_t step, int red,
+                                                               PipelineOut out) {
+  extern __shared__ float s_stage[];
+  __shared__ int64_t s_off[BLOCK + 1];
   const int tid = threadIdx.x;
-  const int64_t L0 = (int64_t)blockIdx.x * kBlock;
-  const int nl = (int)((n - L0) < kBlock ? (n - L0) : kBlock);
-  for (int t = tid; t <= nl; t += kBlock) s_off[t] = offsets[L0 + t];
+  const int64_t L0 = (int64_t)blockIdx.x * BLOCK;
+  const int nl = (int)((n - L0) < BLOCK ? (n - L0) : BLOCK);
+  for (int t = tid; t <= nl; t += BLOCK) s_off[t] = offsets[L0 + t];
   __syncthreads();
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
-  // stage every float32 field when the span fits (others stay in global)
-  const bool staged = span <= kStageFloats;
+  const bool staged = span <= st.cap;
   if (staged) {
     for (int k = 0; k < fields.nfields; k++) {
-      if (fields.type[k] != F32) continue;
+      const int slot = st.slot[k];
+      if (slot < 0) continue;
       const float* src = (const float*)fields.ptr[k];
-      for (int64_t b = 0; b < span; b += (int64_t)kBlock * 4) {
-        float r[4];
+      float* dst = s_stage + slot * st.cap;
+      for (int64_t b = 0; b < span; b += (int64_t)BLOCK * 8) {
+        float r[8];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int64_t q = b + (int64_t)u * kBlock + tid;
+        for (int u = 0; u < 8; u++) {
+          const int64_t q = b + (int64_t)u * BLOCK + tid;
           if (q < span) r[u] = src[E0 + q];
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int64_t q = b + (int64_t)u * kBlock + tid;
-          if (q < span) s_field[k][q] = r[u];
+        for (int u = 0; u < 8; u++) {
+          const int64_t q = b + (int64_t)u * BLOCK + tid;
+          if (q < span) dst[q] = r[u];
         }
       }
     }
@@ -177,34 +204,21 @@ __global__ __launch_bounds__(kBlock) void event_pipeline_kernel(const int64_t* _
   if (tid >= nl) return;
   const int64_t ev = L0 + tid;
   const int64_t a = s_off[tid], b = s_off[tid + 1];
-  const bool red_staged = staged && fields.type[red] == F32;
+  const int red_slot = st.slot[red];
   // pass 1: number of selected particles
   int64_t m = 0;
-  for (int64_t i = a; i < b; i++) {
-    bool keep = true;
-    for (int t = 0; t < pred.nterms && keep; t++) {
-      const Term& term = pred.term[t];
-      keep = eval_term(term, fields, staged ? s_field[term.field] : nullptr, i, i - E0,
-                       staged && fields.type[term.field] == F32);
-    }
-    m += keep ? 1 : 0;
-  }
+  for (int64_t i = a; i < b; i++) m += keep_particle(pred, fields, st, s_stage, staged, i, i - E0) ? 1 : 0;
   int64_t first;
   const int64_t cnt = slice_count(m, start, stop, step, &first);
-  // pass 2: reduce the kept particles (selected-position first + t*step)
+  // pass 2: reduce the kept particles (selected positions first + t*step)
   float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
   int64_t best = -1, t = 0, sel = 0;
   int64_t next = first;
   for (int64_t i = a; i < b && t < cnt; i++) {
-    bool keep = true;
-    for (int q = 0; q < pred.nterms && keep; q++) {
-      const Term& term = pred.term[q];
-      keep = eval_term(term, fields, staged ? s_field[term.field] : nullptr, i, i - E0,
-                       staged && fields.type[term.field] == F32);
-    }
-    if (!keep) continue;
-    if (step > 0 ? sel == next : false) {
-      const float v = red_staged ? s_field[red][i - E0] : ((const float*)fields.ptr[red])[i];
+    if (!keep_particle(pred, fields, st, s_stage, staged, i, i - E0)) continue;
+    if (sel == next) {
+      const float v = (staged && red_slot >= 0) ? s_stage[red_slot * st.cap + (i - E0)]
+                                                : ((const float*)fields.ptr[red])[i];
       sum += v;
       if (t == 0) {
         best = 0;
@@ -237,6 +251,7 @@ __device__ inline float pair_expr(float pt1, float eta1, float phi1, float pt2, 
   return b * (c - d);
 }
 
+constexpr int kBlock = 256;
 constexpr int kPairStage = 2048;
 
 __global__ __launch_bounds__(kBlock) void pair_max_kernel(const int64_t* __restrict__ offsets, int64_t n,
@@ -265,16 +280,19 @@ __global__ __launch_bounds__(kBlock) void pair_max_kernel(const int64_t* __restr
   if (tid >= nl) return;
   const int64_t ev = L0 + tid;
   const int64_t a = s_off[tid], b = s_off[tid + 1];
-  const float* P = staged ? s_pt - E0 : pt;
-  const float* H = staged ? s_eta - E0 : eta;
-  const float* F = staged ? s_phi - E0 : phi;
+  // LDS is indexed with in-range local positions only (a pointer formed
+  // below an LDS array wraps in the 32-bit LDS address space)
+  const int64_t base = staged ? E0 : 0;
+  const float* P = staged ? s_pt : pt;
+  const float* H = staged ? s_eta : eta;
+  const float* F = staged ? s_phi : phi;
   // ak.max: best = identity; best = v > best ? v : best   (awkward_reduce_max.cpp)
   // ak.argmax: first pair seeds, strict improvement only  (awkward_reduce_argmax.cpp)
   float best = identity, argval = 0.0f;
   int64_t arg = -1, r = 0;
-  for (int64_t i = a; i < b; i++) {
+  for (int64_t i = a - base; i < b - base; i++) {
     const float pi = P[i], ei = H[i], fi = F[i];
-    for (int64_t j = i + 1; j < b; j++, r++) {
+    for (int64_t j = i + 1; j < b - base; j++, r++) {
       const float v = pair_expr(pi, ei, fi, P[j], H[j], F[j]);
       best = (v > best) ? v : best;
       if (r == 0) {
@@ -309,9 +327,10 @@ extern "C" {
 // field_ptrs / field_types / term_* are HOST arrays (copied into the kernel
 // arguments).  fields/types: nfields (<= 4) device pointers and type codes (0 f32, 1 f64,
 // 2 i8, 3 i32, 4 i64); terms: nterms (<= 8) of (field, op, absolute, value),
-// ops 0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 !=; red: index of the float32 field that
+// ops 0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 !=; n_particles: the length of the fields
+// (sizes the LDS stage; any value is correct); red: index of the float32 field that
 // is reduced.  Any output pointer may be null.
-Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, const void* const* field_ptrs,
+Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_particles, const void* const* field_ptrs,
                                  const int32_t* field_types, int32_t nfields, const int32_t* term_field,
                                  const int32_t* term_op, const int32_t* term_abs, const double* term_value,
                                  int32_t nterms, int64_t start, int64_t stop, int64_t step, int32_t red,
@@ -340,8 +359,23 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, const void* 
     p.term[t] = Term{term_field[t], term_op[t], term_abs[t], 0, term_value[t]};
   }
   PipelineOut out{out_num, out_sum, out_argmax, out_argmask, out_max, out_maxmask};
-  event_pipeline_kernel<<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream()>>>(offsets, n, f, p, start, stop, step,
-                                                                                red, out);
+  // stage every float32 field the pipeline reads, sized for the workgroup's
+  // expected particle span (mean multiplicity from the array shapes) with
+  // room for its fluctuation; larger spans read global memory directly.
+  Stage st{};
+  st.nslots = 0;
+  for (int k = 0; k < kMaxFields; k++) st.slot[k] = -1;
+  for (int k = 0; k < nfields; k++)
+    if (field_types[k] == F32) st.slot[k] = st.nslots++;
+  constexpr int B = 128;
+  const double mean = n_particles > 0 ? (double)n_particles / (double)n : 0.0;
+  int64_t cap = (int64_t)(B * mean * 1.5) + 256;
+  cap = (cap + 63) / 64 * 64;
+  if (cap > 8192) cap = 8192;
+  st.cap = (int)cap;
+  const size_t lds = (size_t)st.nslots * st.cap * sizeof(float);
+  event_pipeline_kernel<B><<<(unsigned)ceil_div(n, B), B, lds, stream()>>>(offsets, n, f, p, st, start, stop, step,
+                                                                          red, out);
   AKHIP_LAUNCH_CHECK("event_pipeline_kernel");
   return success();
 }

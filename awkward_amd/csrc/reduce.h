@@ -365,10 +365,13 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, typename Op::Out* top
 // is the fixed default of the plain entry points (tiles sized for lists of
 // ~4-30 elements); the host layer picks others from the average list length
 // (elements / lists, known from the array shapes without a device sync), and
-// the autotuner and the learned dispatcher search all of them.
+// the autotuner and the learned dispatcher search all of them.  Small static
+// LDS tiles let more workgroups -- and more loads -- be in flight per CU.
 //   0: 256 x 1, 32 KiB    1: 256 x 4, 8 KiB     2: 256 x 2, 16 KiB
-//   3: 128 x 1, 32 KiB    4: 256 x 1, 64 KiB    5: 256 x 8, 8 KiB
-constexpr int kVariants = 6;
+//   3: 128 x 1, 16 KiB    4: 256 x 1, 64 KiB    5: 256 x 8, 8 KiB
+//   6:  64 x 1,  8 KiB    7: 128 x 2, 16 KiB    8:  64 x 4,  8 KiB
+//   9: 128 x 1,  8 KiB
+constexpr int kVariants = 10;
 
 template <class Op>
 inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
@@ -382,9 +385,13 @@ inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename
   switch (variant) {
     case 1: return launch_reduce_v<Op, 256, 4, 8192>(op, epi, toptr, fromptr, offsets, outlength);
     case 2: return launch_reduce_v<Op, 256, 2, 16384>(op, epi, toptr, fromptr, offsets, outlength);
-    case 3: return launch_reduce_v<Op, 128, 1, 32768>(op, epi, toptr, fromptr, offsets, outlength);
+    case 3: return launch_reduce_v<Op, 128, 1, 16384>(op, epi, toptr, fromptr, offsets, outlength);
     case 4: return launch_reduce_v<Op, 256, 1, 65536>(op, epi, toptr, fromptr, offsets, outlength);
     case 5: return launch_reduce_v<Op, 256, 8, 8192>(op, epi, toptr, fromptr, offsets, outlength);
+    case 6: return launch_reduce_v<Op, 64, 1, 8192>(op, epi, toptr, fromptr, offsets, outlength);
+    case 7: return launch_reduce_v<Op, 128, 2, 16384>(op, epi, toptr, fromptr, offsets, outlength);
+    case 8: return launch_reduce_v<Op, 64, 4, 8192>(op, epi, toptr, fromptr, offsets, outlength);
+    case 9: return launch_reduce_v<Op, 128, 1, 8192>(op, epi, toptr, fromptr, offsets, outlength);
     default: return launch_reduce_v<Op, 256, 1, 32768>(op, epi, toptr, fromptr, offsets, outlength);
   }
 }

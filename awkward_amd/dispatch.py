@@ -3,9 +3,10 @@ library's tuned/fused entry points (``awkward_hip_reduce_*``).
 
 The variants (threads, lists per thread, LDS staging bytes) are the ones
 compiled into libawkward_hip.so (``csrc/reduce.h``, ``launch_reduce``).  The
-fixed heuristic below picks the variant with the most lists per workgroup
-whose expected staged span (lists x mean list length x element size) fills
-at most 85% of the tile's LDS; the mean list length is
+fixed heuristic below models each variant's expected staged span (lists x
+mean list length x element size) and occupancy (workgroups per CU from LDS
+and wave limits) and picks the one with the most bytes in flight per CU
+among those whose stage should hold the span; the mean list length is
 elements / lists, which the host knows from the array shapes without
 synchronising with the device.  ``awkward_amd.tuning`` replaces the
 heuristic by measured choices (the autotuner table, or the learned
@@ -22,29 +23,47 @@ VARIANTS = [
     (256, 1, 32768),
     (256, 4, 8192),
     (256, 2, 16384),
-    (128, 1, 32768),
+    (128, 1, 16384),
     (256, 1, 65536),
     (256, 8, 8192),
+    (64, 1, 8192),
+    (128, 2, 16384),
+    (64, 4, 8192),
+    (128, 1, 8192),
 ]
+
+LDS_PER_CU = 160 * 1024
+WAVES_PER_CU = 32
 
 #: optional override: callable(op, n_lists, n_elements, itemsize) -> variant
 chooser = None
 
 
-def heuristic_variant(n_lists: int, n_elements: int, itemsize: int) -> int:
+def variant_model(v: int, n_lists: int, n_elements: int, itemsize: int):
+    """(expected fill of the LDS stage, expected bytes in flight per CU)."""
+    block, lpt, cap = VARIANTS[v]
     mean = n_elements / max(n_lists, 1)
-    best, best_key = 0, None
-    for v, (block, lpt, cap) in enumerate(VARIANTS):
-        lists = block * lpt
-        fill = lists * mean * itemsize / cap
+    lists = block * lpt
+    staged = lists * mean * itemsize
+    lds = cap + 8 * (lists + 1)
+    per_cu = min(LDS_PER_CU // lds, WAVES_PER_CU // max(block // 64, 1))
+    per_cu = max(per_cu, 1)
+    inflight = per_cu * (min(staged, cap) + 8 * (lists + 1))
+    return staged / cap, inflight
+
+
+def heuristic_variant(n_lists: int, n_elements: int, itemsize: int) -> int:
+    """The variant with the most bytes in flight per CU among those whose
+    stage is expected to hold the workgroup's span (fill <= 85%)."""
+    best, best_key = None, None
+    for v in range(len(VARIANTS)):
+        fill, inflight = variant_model(v, n_lists, n_elements, itemsize)
         if fill > 0.85:
             continue
-        key = (lists, -cap)
+        key = (inflight, -v)
         if best_key is None or key > best_key:
             best, best_key = v, key
-    if best_key is None:  # long lists: the largest staging tile, heavy paths
-        best = 4
-    return best
+    return 4 if best is None else best  # long lists: the largest stage, heavy paths
 
 
 def choose_variant(op: str, n_lists: int, n_elements: int, itemsize: int) -> int:

@@ -73,6 +73,7 @@ class EventPipeline:
         cols = [record._getitem_field(f).data for f in names]
         self.cols = cols
         nplike = record.backend.nplike
+        self.n_particles = nplike.length(cols[0])
         self.field_ptrs = (ctypes.c_void_p * len(cols))(*[c.data_ptr() for c in cols])
         self.field_types = (ctypes.c_int32 * len(cols))(*[_TYPES[nplike.dtype(c)] for c in cols])
         tf, to, ta, tv = [], [], [], []
@@ -100,13 +101,13 @@ class EventPipeline:
                 self.out[key] = torch.empty(self.n, dtype=dt, device=dev)
         P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         self._f = _fn("awkward_hip_event_pipeline",
-                      [P, I64, P, P, I32, P, P, P, P, I32, I64, I64, I64, I32, P, P, P, P, P, P])
+                      [P, I64, I64, P, P, I32, P, P, P, P, I32, I64, I64, I64, I32, P, P, P, P, P, P])
 
     def run(self):
         set_stream()
         tf, to, ta, tv, k = self.terms
         o = self.out
-        _check(self._f(device_address(self.offsets), self.n, ctypes.addressof(self.field_ptrs),
+        _check(self._f(device_address(self.offsets), self.n, self.n_particles, ctypes.addressof(self.field_ptrs),
                        ctypes.addressof(self.field_types), len(self.cols), ctypes.addressof(tf),
                        ctypes.addressof(to), ctypes.addressof(ta), ctypes.addressof(tv), k, self.start, self.stop,
                        self.step, self.red, *[o[k].data_ptr() if k in o else 0 for k in

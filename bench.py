@@ -100,6 +100,15 @@ def run_config1(ak, backend, torch, reps=5):
                            8 * (n + 1) + 8 * N + 9 * n)
     result = {"lists": n, "elements": N, "dtype": "float64", "variant": v,
               "method": "HBM-cold: 1 GiB scrub before each launch; *_tuned = fused entry point, heuristic tile"}
+    from awkward_amd.tuning import time_variants
+
+    sweep = time_variants("sum", out_f, dev_x, dev_off, n, reps=reps, scrub=scrub)
+    sweep_arg = time_variants("argmax", out_i, dev_x, dev_off, n, mask=mask, reps=reps, scrub=scrub)
+    result["sweep_sum_ms"] = {str(k): round(t, 4) for k, t in sweep.items()}
+    result["sweep_argmax_ms"] = {str(k): round(t, 4) for k, t in sweep_arg.items()}
+    vb = min(sweep, key=sweep.get)
+    result["best_variant_sum"] = int(vb)
+    ops["sum_best"] = (lambda: dispatch.reduce_sum(out_f, dev_x, dev_off, n, vb), None, 8 * (n + 1) + 8 * N + 8 * n)
     for name, (key, args, nbytes) in ops.items():
         if args is None:
             call = key

@@ -8,3 +8,16 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+_GPU_ORDER = ["test_hip_golden", "test_hip_parity", "test_pipeline", "test_fused"]
+
+
+def pytest_collection_modifyitems(session, config, items):
+    """GPU files run core kernel parity first, fused/graph forms last."""
+
+    def rank(item):
+        mod = item.module.__name__.rsplit(".", 1)[-1]
+        return _GPU_ORDER.index(mod) if mod in _GPU_ORDER else -1
+
+    items.sort(key=rank)

@@ -317,7 +317,7 @@ def test_local_preparenext():
     assert same_bits(h[0], d[0])
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(10))
 @pytest.mark.parametrize("t", ["int32", "int64", "float32", "float64"])
 def test_tuned_variants(variant, t):
     """The tuned/fused entry points, every tile variant, against the oracle's
