@@ -361,9 +361,8 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, typename Op::Out* top
   return success();
 }
 
-// Tile variants (threads, lists per thread, LDS staging bytes).  Variant 0
-// is the fixed default of the plain entry points (tiles sized for lists of
-// ~4-30 elements); the host layer picks others from the average list length
+// Tile variants (threads, lists per thread, LDS staging bytes).  The host
+// layer picks one from the average list length
 // (elements / lists, known from the array shapes without a device sync), and
 // the autotuner and the learned dispatcher search all of them.  Small static
 // LDS tiles let more workgroups -- and more loads -- be in flight per CU.
@@ -373,10 +372,12 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, typename Op::Out* top
 //   9: 128 x 1,  8 KiB
 constexpr int kVariants = 10;
 
+// The plain entry points use variant 3 (128 threads, one list each, 16 KiB):
+// the fastest tile for lists of ~4-30 elements at every size measured.
 template <class Op>
 inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
                            const int64_t* offsets, int64_t outlength) {
-  return launch_reduce_v<Op, 256, 1, 32768>(op, PlainEpilogue{}, toptr, fromptr, offsets, outlength);
+  return launch_reduce_v<Op, 128, 1, 16384>(op, PlainEpilogue{}, toptr, fromptr, offsets, outlength);
 }
 
 template <class Op, class Epi>
