@@ -79,22 +79,6 @@ __device__ inline bool compare(T x, int op, T c) {
   }
 }
 
-// One term against element i of its field (float32 fields compare in
-// float32 against the constant rounded to float32, as numpy's weak-scalar
-// rule and torch's scalar comparisons do).
-__device__ inline bool eval_term(const Term& t, const Fields& f, const float* s_f32, int64_t i, int64_t local,
-                                 bool staged) {
-  const int type = f.type[t.field];
-  if (type == F32) {
-    float x = staged ? s_f32[local] : ((const float*)f.ptr[t.field])[i];
-    if (t.absolute) x = fabsf(x);
-    return compare<float>(x, t.op, (float)t.value);
-  }
-  double x = load_as_double(f.ptr[t.field], type, i);
-  if (t.absolute) x = fabs(x);
-  return compare<double>(x, t.op, t.value);
-}
-
 // awkward_regularize_rangeslice (kernel-utils.cpp:7-39) -> (first, count)
 __device__ inline int64_t slice_count(int64_t length, int64_t start, int64_t stop, int64_t step, int64_t* first) {
   const bool hasstart = start != kSliceNone, hasstop = stop != kSliceNone;
@@ -110,6 +94,7 @@ __device__ inline int64_t slice_count(int64_t length, int64_t start, int64_t sto
     if (b > length) b = length;
     if (b < a) b = a;
     *first = a;
+    if (step == 1) return b - a;
     return (b > a) ? 1 + (b - a - 1) / step : 0;
   }
   if (!hasstart) a = length - 1;
@@ -143,102 +128,178 @@ struct Stage {
   int cap;
 };
 
-__device__ inline bool keep_particle(const Predicate& pred, const Fields& fields, const Stage& st,
-                                     const float* s_stage, bool staged, int64_t i, int64_t local) {
-  for (int t = 0; t < pred.nterms; t++) {
-    const Term& term = pred.term[t];
-    const int slot = st.slot[term.field];
-    bool ok;
-    if (staged && slot >= 0) {
-      float x = s_stage[slot * st.cap + local];
-      if (term.absolute) x = fabsf(x);
-      ok = compare<float>(x, term.op, (float)term.value);
-    } else {
-      ok = eval_term(term, fields, nullptr, i, local, false);
-    }
-    if (!ok) return false;
+// A term resolved once per thread before the particle loops: its LDS row
+// (or global column), type, operator and constant live in registers, so the
+// loops never re-read kernel arguments.
+struct TermRT {
+  const float* lds;  // staged float32 row, or null
+  const void* gptr;  // global column
+  int type;
+  int op;
+  int absolute;
+  float fval;
+  double dval;
+};
+
+__device__ inline bool eval_rt(const TermRT& t, int64_t i, int64_t local) {
+  if (t.lds) {
+    float x = t.lds[local];
+    if (t.absolute) x = fabsf(x);
+    return compare<float>(x, t.op, t.fval);
   }
+  // float32 compares in float32 against the constant rounded to float32, as
+  // numpy's weak-scalar rule and torch's scalar comparisons do
+  if (t.type == F32) {
+    float x = ((const float*)t.gptr)[i];
+    if (t.absolute) x = fabsf(x);
+    return compare<float>(x, t.op, t.fval);
+  }
+  double x = load_as_double(t.gptr, t.type, i);
+  if (t.absolute) x = fabs(x);
+  return compare<double>(x, t.op, t.dval);
+}
+
+// NT (the number of terms) is a template parameter so the unrolled loops
+// index the resolved terms statically (registers, no local memory).
+template <int NT>
+__device__ inline bool keep_particle(const TermRT* rt, int64_t i, int64_t local) {
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+    if (!eval_rt(rt[t], i, local)) return false;
   return true;
 }
 
-// BLOCK threads, one event each; the workgroup's particles are staged with
-// coalesced loads into a stage sized at launch (dynamic LDS) to keep many
-// workgroups -- and many loads -- in flight per CU.
-template <int BLOCK>
+// BLOCK threads, LPT events each (thread t owns events t + j * BLOCK, so
+// output stores coalesce); the workgroup's particles are staged with
+// coalesced loads -- every staged field's loads issued together -- into a
+// stage sized at launch (dynamic LDS), to keep many loads in flight per CU.
+template <int BLOCK, int LPT, int NT>
 __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
                                                                Fields fields, Predicate pred, Stage st,
                                                                int64_t start, int64_t stop, int64_t step, int red,
                                                                PipelineOut out) {
+  constexpr int L = BLOCK * LPT;
+  constexpr int U = 4;
   extern __shared__ float s_stage[];
-  __shared__ int64_t s_off[BLOCK + 1];
+  __shared__ int64_t s_off[L + 1];
   const int tid = threadIdx.x;
-  const int64_t L0 = (int64_t)blockIdx.x * BLOCK;
-  const int nl = (int)((n - L0) < BLOCK ? (n - L0) : BLOCK);
-  for (int t = tid; t <= nl; t += BLOCK) s_off[t] = offsets[L0 + t];
+  const int64_t L0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)((n - L0) < L ? (n - L0) : L);
+  {
+    int64_t r[LPT + 1];
+#pragma unroll
+    for (int j = 0; j <= LPT; j++) {
+      const int t = tid + j * BLOCK;
+      if (t <= nl) r[j] = offsets[L0 + t];
+    }
+#pragma unroll
+    for (int j = 0; j <= LPT; j++) {
+      const int t = tid + j * BLOCK;
+      if (t <= nl) s_off[t] = r[j];
+    }
+  }
   __syncthreads();
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
   const bool staged = span <= st.cap;
   if (staged) {
-    for (int k = 0; k < fields.nfields; k++) {
-      const int slot = st.slot[k];
-      if (slot < 0) continue;
-      const float* src = (const float*)fields.ptr[k];
-      float* dst = s_stage + slot * st.cap;
-      for (int64_t b = 0; b < span; b += (int64_t)BLOCK * 8) {
-        float r[8];
+    const float* src[kMaxFields];
+    float* dst[kMaxFields];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int64_t q = b + (int64_t)u * BLOCK + tid;
-          if (q < span) r[u] = src[E0 + q];
+    for (int k = 0; k < kMaxFields; k++) {
+      const int slot = k < fields.nfields ? st.slot[k] : -1;
+      src[k] = slot >= 0 ? (const float*)fields.ptr[k] + E0 : nullptr;
+      dst[k] = slot >= 0 ? s_stage + slot * st.cap : nullptr;
+    }
+    const int sp = (int)span;
+    for (int b = 0; b < sp; b += BLOCK * U) {
+      float r[kMaxFields][U];
+#pragma unroll
+      for (int k = 0; k < kMaxFields; k++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = b + u * BLOCK + tid;
+          if (src[k] && q < sp) r[k][u] = src[k][q];
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int64_t q = b + (int64_t)u * BLOCK + tid;
-          if (q < span) dst[q] = r[u];
+      for (int k = 0; k < kMaxFields; k++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = b + u * BLOCK + tid;
+          if (dst[k] && q < sp) dst[k][q] = r[k][u];
         }
-      }
     }
   }
   __syncthreads();
-  if (tid >= nl) return;
-  const int64_t ev = L0 + tid;
-  const int64_t a = s_off[tid], b = s_off[tid + 1];
-  const int red_slot = st.slot[red];
-  // pass 1: number of selected particles
-  int64_t m = 0;
-  for (int64_t i = a; i < b; i++) m += keep_particle(pred, fields, st, s_stage, staged, i, i - E0) ? 1 : 0;
-  int64_t first;
-  const int64_t cnt = slice_count(m, start, stop, step, &first);
-  // pass 2: reduce the kept particles (selected positions first + t*step)
-  float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
-  int64_t best = -1, t = 0, sel = 0;
-  int64_t next = first;
-  for (int64_t i = a; i < b && t < cnt; i++) {
-    if (!keep_particle(pred, fields, st, s_stage, staged, i, i - E0)) continue;
-    if (sel == next) {
-      const float v = (staged && red_slot >= 0) ? s_stage[red_slot * st.cap + (i - E0)]
-                                                : ((const float*)fields.ptr[red])[i];
-      sum += v;
-      if (t == 0) {
-        best = 0;
-        best_val = v;
-      } else if (v > best_val) {
-        best_val = v;
-        best = t;
-      }
-      maxv = (v > maxv) ? v : maxv;
-      t++;
-      next += step;
-    }
-    sel++;
+  TermRT rt[NT > 0 ? NT : 1];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const Term& term = pred.term[t];
+    const int slot = st.slot[term.field];
+    rt[t].lds = (staged && slot >= 0) ? s_stage + slot * st.cap : nullptr;
+    rt[t].gptr = fields.ptr[term.field];
+    rt[t].type = fields.type[term.field];
+    rt[t].op = term.op;
+    rt[t].absolute = term.absolute;
+    rt[t].fval = (float)term.value;
+    rt[t].dval = term.value;
   }
-  if (out.num) out.num[ev] = cnt;
-  if (out.sum) out.sum[ev] = sum;
-  if (out.argmax) out.argmax[ev] = best;
-  if (out.argmask) out.argmask[ev] = cnt > 0 ? 0 : 1;
-  if (out.max) out.max[ev] = maxv;
-  if (out.maxmask) out.maxmask[ev] = cnt > 0 ? 0 : 1;
+  // the predicate, once per particle, element-parallel (no divergence):
+  // one flag byte per staged particle, after the float rows (a workgroup
+  // scan of the flags into ranks measured slower than the per-event count)
+  uint8_t* s_flag = (uint8_t*)(s_stage + st.nslots * st.cap);
+  if (staged) {
+    for (int q = tid; q < (int)span; q += BLOCK) s_flag[q] = keep_particle<NT>(rt, E0 + q, q) ? 1 : 0;
+  }
+  __syncthreads();
+  const int red_slot = st.slot[red];
+  const float* red_lds = (staged && red_slot >= 0) ? s_stage + red_slot * st.cap : nullptr;
+  const float* red_g = (const float*)fields.ptr[red];
+#pragma unroll
+  for (int j = 0; j < LPT; j++) {
+    const int e = tid + j * BLOCK;
+    if (e >= nl) break;
+    const int64_t ev = L0 + e;
+    const int64_t a = s_off[e], b = s_off[e + 1];
+    // pass 1: number of selected particles
+    int64_t m = 0;
+    if (staged) {
+      for (int q = (int)(a - E0); q < (int)(b - E0); q++) m += s_flag[q];
+    } else {
+      for (int64_t i = a; i < b; i++) m += keep_particle<NT>(rt, i, i - E0) ? 1 : 0;
+    }
+    int64_t first;
+    const int64_t cnt = slice_count(m, start, stop, step, &first);
+    // pass 2: reduce the kept particles (selected positions first + t*step)
+    float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
+    int64_t best = -1, t = 0, sel = 0;
+    int64_t next = first;
+    for (int64_t i = a; i < b && t < cnt; i++) {
+      const bool keep = staged ? s_flag[i - E0] != 0 : keep_particle<NT>(rt, i, i - E0);
+      if (!keep) continue;
+      if (sel == next) {
+        const float v = red_lds ? red_lds[i - E0] : red_g[i];
+        sum += v;
+        if (t == 0) {
+          best = 0;
+          best_val = v;
+        } else if (v > best_val) {
+          best_val = v;
+          best = t;
+        }
+        maxv = (v > maxv) ? v : maxv;
+        t++;
+        next += step;
+      }
+      sel++;
+    }
+    if (out.num) out.num[ev] = cnt;
+    if (out.sum) out.sum[ev] = sum;
+    if (out.argmax) out.argmax[ev] = best;
+    if (out.argmask) out.argmask[ev] = cnt > 0 ? 0 : 1;
+    if (out.max) out.max[ev] = maxv;
+    if (out.maxmask) out.maxmask[ev] = cnt > 0 ? 0 : 1;
+  }
 }
 
 // ---- pairs ---------------------------------------------------------------------
@@ -367,15 +428,34 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_pa
   for (int k = 0; k < kMaxFields; k++) st.slot[k] = -1;
   for (int k = 0; k < nfields; k++)
     if (field_types[k] == F32) st.slot[k] = st.nslots++;
+  // events per thread: at least ~4 KiB of staged particles per workgroup
+  // (one event per thread at the configs' multiplicities, which measured best)
   constexpr int B = 128;
   const double mean = n_particles > 0 ? (double)n_particles / (double)n : 0.0;
-  int64_t cap = (int64_t)(B * mean * 1.5) + 256;
+  const double per_event = mean * (st.nslots > 0 ? st.nslots : 1) * sizeof(float) * 1.5;
+  int lpt = 1;
+  while (lpt < 8 && (double)B * (2 * lpt) * per_event <= 4096.0) lpt *= 2;
+  int64_t cap = (int64_t)(B * lpt * mean * 1.5) + 256;
   cap = (cap + 63) / 64 * 64;
-  if (cap > 8192) cap = 8192;
+  if (cap > 12288) cap = 12288;
   st.cap = (int)cap;
-  const size_t lds = (size_t)st.nslots * st.cap * sizeof(float);
-  event_pipeline_kernel<B><<<(unsigned)ceil_div(n, B), B, lds, stream()>>>(offsets, n, f, p, st, start, stop, step,
-                                                                          red, out);
+  const size_t lds = (size_t)st.nslots * st.cap * sizeof(float) + (size_t)st.cap;  // + predicate flags
+  const unsigned grid = (unsigned)ceil_div(n, (int64_t)B * lpt);
+  hipStream_t s = stream();
+  switch (lpt * 16 + nterms) {
+#define CASE(LPT, K)                                                                                              \
+  case LPT * 16 + K:                                                                                              \
+    event_pipeline_kernel<B, LPT, K><<<grid, B, lds, s>>>(offsets, n, f, p, st, start, stop, step, red, out); \
+    break;
+#define CASES(LPT) CASE(LPT, 0) CASE(LPT, 1) CASE(LPT, 2) CASE(LPT, 3) CASE(LPT, 4) CASE(LPT, 5) CASE(LPT, 6) \
+    CASE(LPT, 7) CASE(LPT, 8)
+    CASES(1)
+    CASES(2)
+    CASES(4)
+    CASES(8)
+#undef CASES
+#undef CASE
+  }
   AKHIP_LAUNCH_CHECK("event_pipeline_kernel");
   return success();
 }
