@@ -1,0 +1,215 @@
+// batch.hip -- the batch runner's persistent work-queue kernels (no
+// counterpart in awkward-cpp).
+//
+// A batch is many independent jagged arrays of one dtype (zero-based int64
+// offsets, contiguous content).  Each array is cut into tiles of lists; the
+// tiles of all arrays are numbered consecutively (tile_begin per array,
+// computed on the host from the list counts) and a fixed grid of persistent
+// workgroups pulls tile numbers from an atomic counter until the queue is
+// empty -- so uneven arrays balance themselves across the chip and the whole
+// batch is one launch per dtype and op, capturable in a hipGraph.
+//
+//   awkward_hip_batch_reduce_<T>: per list, sum (awkward_reduce_sum's output
+//     type), min (awkward_reduce_min with its identity) and the local argmin
+//     with the empty-list mask (awkward_reduce_argmin + adjust_starts +
+//     reduce_mask), all from one pass over the elements;
+//   awkward_hip_batch_argsort_<T>: per list, the stable argsort
+//     (awkward_argsort), local indices.
+//
+// The counter lives in device memory that the caller clears before each
+// launch (awkward_hip_batch_reset: a kernel, so the sequence can be graph
+// captured).  Every workgroup leaves the loop when the counter passes the
+// last tile, so the grid always drains.
+
+#include "reduce.h"
+#include "sort.h"
+
+namespace akhip {
+namespace {
+
+struct BatchReduceItem {
+  const void* data;
+  const int64_t* offsets;
+  void* out_sum;
+  void* out_min;
+  int64_t* out_argmin;
+  int8_t* out_mask;
+  int64_t n;
+  int64_t tile_begin;
+};
+
+struct BatchSortItem {
+  const void* data;
+  const int64_t* offsets;
+  int64_t* out;
+  int64_t n;         // lists
+  int64_t length;    // elements
+  int64_t tile_begin;
+  int64_t elem_begin;  // this array's region of the merge buffers
+};
+
+template <typename T, typename S>
+struct BatchEpilogue {
+  S* sum;
+  T* mn;
+  int64_t* arg;
+  int8_t* mask;
+  __device__ void store(int64_t bin, MultiOut<T, S> v, int64_t gb, int64_t ge) const {
+    sum[bin] = v.sum;
+    mn[bin] = v.min;
+    arg[bin] = v.arg >= 0 ? v.arg - gb : -1;
+    mask[bin] = (ge > gb) ? 0 : 1;
+  }
+};
+
+// item owning global tile `tile`: the last one with tile_begin <= tile
+template <typename Item>
+__device__ inline int find_item(const Item* items, int nitems, int64_t tile) {
+  int lo = 0, hi = nitems;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (items[mid].tile_begin <= tile) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void reset_counter(unsigned long long* counter) { *counter = 0ull; }
+
+template <typename T, typename S, int BLOCK, int LPT, int CAP>
+__global__ __launch_bounds__(BLOCK) void batch_reduce_kernel(const BatchReduceItem* __restrict__ items, int nitems,
+                                                             int64_t total_tiles, unsigned long long* counter,
+                                                             T identity) {
+  using Op = SumMinArgminOp<T, S>;
+  __shared__ ReduceSmem<Op, BLOCK, LPT, CAP> sm;
+  __shared__ int64_t s_tile;
+  __shared__ int s_item;
+  const Op op{identity};
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int64_t t = (int64_t)atomicAdd(counter, 1ull);
+      s_tile = t;
+      s_item = t < total_tiles ? find_item(items, nitems, t) : 0;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= total_tiles) break;
+    const BatchReduceItem it = items[s_item];
+    __syncthreads();  // s_tile / s_item are rewritten by the next claim
+    const BatchEpilogue<T, S> epi{(S*)it.out_sum, (T*)it.out_min, it.out_argmin, it.out_mask};
+    reduce_tile<Op, BLOCK, LPT, CAP>(op, epi, (const T*)it.data, it.offsets, it.n,
+                                     (tile - it.tile_begin) * (int64_t)(BLOCK * LPT), sm);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(sortimpl::kBlock) void batch_argsort_kernel(const BatchSortItem* __restrict__ items,
+                                                                         int nitems, int64_t total_tiles,
+                                                                         unsigned long long* counter, bool asc,
+                                                                         uint64_t* gk0, uint32_t* gi0, uint64_t* gk1,
+                                                                         uint32_t* gi1) {
+  using namespace sortimpl;
+  __shared__ SortSmem sm;
+  __shared__ int64_t s_tile;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int64_t t = (int64_t)atomicAdd(counter, 1ull);
+      s_tile = t;
+      s_item = t < total_tiles ? find_item(items, nitems, t) : 0;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= total_tiles) break;
+    const BatchSortItem it = items[s_item];
+    __syncthreads();
+    const int64_t e0 = it.elem_begin;
+    sort_tile<T, int64_t, true>(it.out, (const T*)it.data, it.length, it.offsets, it.n,
+                                (tile - it.tile_begin) * (int64_t)kBlock, asc, gk0 + e0, gi0 + e0, gk1 + e0,
+                                gi1 + e0, sm);
+  }
+}
+
+template <typename T, typename S, int BLOCK, int LPT, int CAP>
+Error launch_batch_reduce(const BatchReduceItem* items, int nitems, int64_t total_tiles, unsigned long long* counter,
+                          T identity) {
+  // persistent grid: a few workgroups per CU, never more than the tiles
+  int64_t grid = (int64_t)cu_count() * (BLOCK <= 64 ? 16 : BLOCK <= 128 ? 8 : 4);
+  if (grid > total_tiles) grid = total_tiles;
+  if (grid < 1) grid = 1;
+  batch_reduce_kernel<T, S, BLOCK, LPT, CAP><<<(unsigned)grid, BLOCK, 0, stream()>>>(items, nitems, total_tiles,
+                                                                                     counter, identity);
+  AKHIP_LAUNCH_CHECK("batch_reduce_kernel");
+  return success();
+}
+
+}  // namespace
+}  // namespace akhip
+
+using namespace akhip;
+
+extern "C" {
+
+// Lists per tile of each batch-reduce variant (the host numbers the tiles).
+int64_t awkward_hip_batch_reduce_tile_lists(int variant) {
+  switch (variant) {
+    case 1: return 64;
+    case 2: return 1024;
+    case 3: return 2048;
+    case 4: return 256;
+    default: return 128;
+  }
+}
+
+int64_t awkward_hip_batch_sort_tile_lists(void) { return sortimpl::kBlock; }
+
+Error awkward_hip_batch_reset(unsigned long long* counter) {
+  reset_counter<<<1, 1, 0, stream()>>>(counter);
+  AKHIP_LAUNCH_CHECK("reset_counter");
+  return success();
+}
+
+#define BATCH_REDUCE(TN, T, S)                                                                                   \
+  Error awkward_hip_batch_reduce_##TN(const void* items, int32_t nitems, int64_t total_tiles,                   \
+                                      unsigned long long* counter, T identity, int32_t variant) {              \
+    const BatchReduceItem* it = (const BatchReduceItem*)items;                                                 \
+    if (nitems <= 0 || total_tiles <= 0) return success();                                                     \
+    switch (variant) {                                                                                         \
+      case 1: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, total_tiles, counter, identity);       \
+      case 2: return launch_batch_reduce<T, S, 256, 4, 8192>(it, nitems, total_tiles, counter, identity);      \
+      case 3: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, total_tiles, counter, identity);      \
+      case 4: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, total_tiles, counter, identity);     \
+      default: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, total_tiles, counter, identity);    \
+    }                                                                                                          \
+  }
+BATCH_REDUCE(int32, int32_t, int64_t)
+BATCH_REDUCE(int64, int64_t, int64_t)
+BATCH_REDUCE(float32, float, float)
+BATCH_REDUCE(float64, double, double)
+#undef BATCH_REDUCE
+
+// scratch: 12 bytes x (total elements of the batch) x 2, from the pool.
+#define BATCH_SORT(TN, T)                                                                                        \
+  Error awkward_hip_batch_argsort_##TN(const void* items, int32_t nitems, int64_t total_tiles,                  \
+                                       unsigned long long* counter, bool ascending, int64_t total_elements) {  \
+    if (nitems <= 0 || total_tiles <= 0) return success();                                                    \
+    const size_t kb = ((size_t)(total_elements > 0 ? total_elements : 1) * 8 + 255) / 256 * 256;              \
+    const size_t ib = ((size_t)(total_elements > 0 ? total_elements : 1) * 4 + 255) / 256 * 256;              \
+    char* s = (char*)scratch(2 * (kb + ib));                                                                   \
+    if (!s) return failure("out of device memory for batch sort scratch", kSliceNone, kSliceNone,              \
+                           "awkward_amd/csrc/batch.hip");                                                      \
+    int64_t grid = (int64_t)cu_count() * 4;                                                                    \
+    if (grid > total_tiles) grid = total_tiles;                                                                \
+    batch_argsort_kernel<T><<<(unsigned)grid, sortimpl::kBlock, 0, stream()>>>(                                \
+        (const BatchSortItem*)items, nitems, total_tiles, counter, ascending, (uint64_t*)s, (uint32_t*)(s + 2 * kb), \
+        (uint64_t*)(s + kb), (uint32_t*)(s + 2 * kb + ib));                                                    \
+    AKHIP_LAUNCH_CHECK("batch_argsort_kernel");                                                                \
+    return success();                                                                                          \
+  }
+BATCH_SORT(int32, int32_t)
+BATCH_SORT(int64, int64_t)
+BATCH_SORT(float32, float)
+BATCH_SORT(float64, double)
+#undef BATCH_SORT
+
+}  // extern "C"

@@ -20,6 +20,7 @@ namespace akhip {
 namespace {
 
 const char* kFile = "awkward_amd/csrc/combinations.hip";
+constexpr int kMaxN = 8;  // tuple sizes decoded on the device
 
 // C(m, k) by the CPU kernel's incremental multiply-then-divide
 // (awkward_ListArray_combinations_length.cpp:16-43), for m >= 0.
@@ -43,12 +44,110 @@ __host__ __device__ inline int64_t comb_count(int64_t size, int64_t n, bool repl
   return choose(size, n);
 }
 
+// C(m, K) for a compile-time K: the same incremental product, but every
+// divisor is a constant, so no runtime 64-bit division is emitted.
+template <int K>
+__device__ inline int64_t choose_k(int64_t m) {
+  if (m < K) return 0;
+  int64_t c = 1;
+#pragma unroll
+  for (int j = 1; j <= K; j++) c = c * (m - j + 1) / j;  // exact at every step
+  return c;
+}
+
+__device__ inline int64_t choose_small(int64_t m, int64_t k) {
+  switch (k) {
+    case 0: return 1;
+    case 1: return m > 0 ? m : 0;
+    case 2: return choose_k<2>(m);
+    case 3: return choose_k<3>(m);
+    case 4: return choose_k<4>(m);
+    case 5: return choose_k<5>(m);
+    case 6: return choose_k<6>(m);
+    case 7: return choose_k<7>(m);
+    default: return choose_k<8>(m);
+  }
+}
+
 // Tuples whose remaining kk positions are drawn from [x, s): C(s-x, kk), or
-// the multiset count C(s-x+kk-1, kk) with replacement.
+// the multiset count C(s-x+kk-1, kk) with replacement (kk <= 8).
 __device__ inline int64_t tail_count(int64_t s, int64_t x, int64_t kk, bool replacement) {
   const int64_t m = s - x;
   if (m <= 0) return 0;
-  return replacement ? choose(m + kk - 1, kk) : choose(m, kk);
+  return replacement ? choose_small(m + kk - 1, kk) : choose_small(m, kk);
+}
+
+// Lexicographic unranking of n-combinations of [0, s).  Multisets (with
+// replacement) map to combinations of s + n - 1 items by x_i' = x_i + i, an
+// order-preserving bijection, so both use the same unrankers.
+//
+// pairs: the first element a is the largest with F(a) = a(2s - a - 1)/2 <= r
+// (pairs starting before a); estimated from the quadratic, then corrected
+// with exact integer counts.
+__device__ inline void unrank2(int64_t r, int64_t s, int64_t* x) {
+  const double t = 2.0 * (double)s - 1.0;
+  double disc = t * t - 8.0 * (double)r;
+  if (disc < 0.0) disc = 0.0;
+  int64_t a = (int64_t)((t - sqrt(disc)) * 0.5);
+  if (a < 0) a = 0;
+  if (a > s - 2) a = s - 2;
+  auto F = [s](int64_t q) { return q * (2 * s - q - 1) / 2; };
+  while (a > 0 && F(a) > r) a--;
+  while (a < s - 2 && F(a + 1) <= r) a++;
+  x[0] = a;
+  x[1] = a + 1 + (r - F(a));
+}
+
+// triples: with R = C(s,3) - r, the first element is a = s - m for the
+// smallest m with C(m, 3) >= R (m ~ cbrt(6R)); the rest is a pair of the
+// m - 1 items after a.
+__device__ inline void unrank3(int64_t r, int64_t s, int64_t* x) {
+  auto C3 = [](int64_t m) { return m < 3 ? (int64_t)0 : m * (m - 1) * (m - 2) / 6; };
+  const int64_t R = C3(s) - r;
+  int64_t m = (int64_t)cbrt(6.0 * (double)R) + 1;
+  if (m < 3) m = 3;
+  if (m > s) m = s;
+  while (m > 3 && C3(m - 1) >= R) m--;
+  while (m < s && C3(m) < R) m++;
+  const int64_t a = s - m;
+  const int64_t r2 = r - (C3(s) - C3(m));
+  int64_t y[2];
+  unrank2(r2, m - 1, y);
+  x[0] = a;
+  x[1] = a + 1 + y[0];
+  x[2] = a + 1 + y[1];
+}
+
+// any n <= kMaxN: one binary search per position over closed-form counts
+__device__ inline void unrank_general(int64_t r, int64_t s, int64_t n, int64_t* x) {
+  int64_t prev = -1;
+  for (int64_t q = 0; q < n; q++) {
+    const int64_t kk = n - q;
+    const int64_t xlo = prev + 1, xhi = s - kk;
+    const int64_t total = tail_count(s, xlo, kk, false);
+    int64_t a = xlo, b = xhi;
+    while (a < b) {
+      const int64_t mid = (a + b + 1) >> 1;
+      if (total - tail_count(s, mid, kk, false) <= r) a = mid;
+      else b = mid - 1;
+    }
+    r -= total - tail_count(s, a, kk, false);
+    x[q] = a;
+    prev = a;
+  }
+}
+
+template <int N>
+__device__ inline void unrank(int64_t r, int64_t s, int64_t n, bool replacement, int64_t* x) {
+  const int64_t sz = replacement ? s + n - 1 : s;
+  if constexpr (N == 2) unrank2(r, sz, x);
+  else if constexpr (N == 3) unrank3(r, sz, x);
+  else unrank_general(r, sz, n, x);
+  if (replacement) {
+#pragma unroll
+    for (int q = 1; q < (N > 0 ? N : kMaxN); q++)
+      if (q < n) x[q] -= q;
+  }
 }
 
 template <typename C>
@@ -77,11 +176,10 @@ __global__ void zero_two(int64_t* a, int64_t* b) {
 
 constexpr int kDecodeBlock = 256;
 constexpr int kDecodeSlots = 4;  // slots per thread per tile
-constexpr int kMaxN = 8;         // tuple sizes decoded on the device
 
 // Persistent decode over P = offsets[length] slots (read on the device, so
 // no host synchronisation is needed between the scan and the decode).
-template <typename C>
+template <typename C, int N>
 __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __restrict__ tocarry,
                                                               int64_t* __restrict__ toindex,
                                                               const int64_t* __restrict__ offsets,
@@ -96,12 +194,25 @@ __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __
   if (threadIdx.x < n) s_out[threadIdx.x] = tocarry[threadIdx.x];
   const int64_t P = offsets[length];
   if (blockIdx.x == 0 && threadIdx.x < n) toindex[threadIdx.x] = P;
-  for (int64_t tile = blockIdx.x; tile * TILE < P; tile += gridDim.x) {
+  // each workgroup decodes a contiguous run of tiles, so the next tile's
+  // lists are found by galloping forward from the previous tile's last list
+  const int64_t ntiles = (P + TILE - 1) / TILE;
+  const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const int64_t t_begin = (int64_t)blockIdx.x * per;
+  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  int64_t hint = 0;  // a list with offsets[hint] <= the next tile's first slot
+  for (int64_t tile = t_begin; tile < t_end; tile++) {
     const int64_t p0 = tile * TILE;
     const int64_t p1 = (p0 + TILE < P ? p0 + TILE : P) - 1;  // last slot of the tile
     if (threadIdx.x < 2) {
       const int64_t target = threadIdx.x == 0 ? p0 : p1;
-      int64_t lo = 0, hi = length;  // offsets[lo] <= target < offsets[hi]
+      // largest l with offsets[l] <= target, from hint: gallop, then bisect
+      int64_t lo = hint, step = 1;
+      while (lo + step < length && offsets[lo + step] <= target) {
+        lo += step;
+        step <<= 1;
+      }
+      int64_t hi = lo + step < length ? lo + step : length;  // offsets[hi] > target or hi == length
       while (hi - lo > 1) {
         const int64_t mid = (lo + hi) >> 1;
         if (offsets[mid] <= target) lo = mid;
@@ -111,6 +222,7 @@ __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __
     }
     __syncthreads();
     const int64_t l0 = s_range[0], l1 = s_range[1];
+    hint = l1;
     const int64_t nl = l1 - l0 + 1;
     const bool cached = nl <= kCache;
     if (cached)
@@ -141,24 +253,11 @@ __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __
       const int64_t l = l0 + lo;
       const int64_t start = (int64_t)starts[l];
       const int64_t s = (int64_t)stops[l] - start;
-      int64_t r = p - base;
-      int64_t prev = replacement ? 0 : -1;
-      for (int64_t q = 0; q < n; q++) {
-        const int64_t kk = n - q;
-        const int64_t xlo = replacement ? prev : prev + 1;
-        const int64_t xhi = replacement ? s - 1 : s - kk;
-        const int64_t total = tail_count(s, xlo, kk, replacement);
-        // largest x in [xlo, xhi] with total - tail_count(s, x, kk) <= r
-        int64_t a = xlo, b = xhi;
-        while (a < b) {
-          const int64_t mid = (a + b + 1) >> 1;
-          if (total - tail_count(s, mid, kk, replacement) <= r) a = mid;
-          else b = mid - 1;
-        }
-        r -= total - tail_count(s, a, kk, replacement);
-        s_out[q][p] = start + a;
-        prev = a;
-      }
+      int64_t x[kMaxN];
+      unrank<N>(p - base, s, n, replacement, x);
+#pragma unroll
+      for (int q = 0; q < (N > 0 ? N : kMaxN); q++)
+        if (q < n) s_out[q][p] = start + x[q];
     }
     __syncthreads();  // s_off and s_range are rewritten by the next tile
   }
@@ -185,8 +284,11 @@ Error launch_decode(int64_t** tocarry, int64_t* toindex, const int64_t* offsets,
                     int64_t length, int64_t n, bool replacement, int64_t grid_hint) {
   int64_t grid = (int64_t)cu_count() * 16;
   if (grid_hint > 0 && grid_hint < grid) grid = grid_hint;
-  decode_kernel<C><<<(unsigned)grid, kDecodeBlock, 0, stream()>>>(tocarry, toindex, offsets, starts, stops, length,
-                                                                   n, replacement);
+  const unsigned g = (unsigned)grid;
+  hipStream_t st = stream();
+  if (n == 2) decode_kernel<C, 2><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
+  else if (n == 3) decode_kernel<C, 3><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
+  else decode_kernel<C, 0><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
   AKHIP_LAUNCH_CHECK("combinations decode_kernel");
   return success();
 }

@@ -195,35 +195,98 @@ struct ArgOp {
   __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_vi(p, m); }
 };
 
-// ---- the kernel -----------------------------------------------------------------
-// Epilogue applied to each list's result before the store (identity for the
-// plain kernels; the fused host-layer variants subtract the list start and
-// write the empty-list mask).
-struct PlainEpilogue {
-  template <typename Out>
-  __device__ Out apply(Out v, int64_t, int64_t, int64_t) const { return v; }
+// sum, min and argmin of each list from one pass over its elements, with
+// the semantics of awkward_reduce_sum, awkward_reduce_min and
+// awkward_reduce_argmin (the batch runner's fused op).
+template <typename T, typename S>
+struct MultiOut {
+  S sum;
+  T min;
+  int64_t arg;  // global index, -1 for an empty list
 };
 
-template <class Op, int BLOCK, int LPT, int CAP_BYTES, class Epi = PlainEpilogue>
-__global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi, typename Op::Out* __restrict__ toptr,
-                                                                 const typename Op::In* __restrict__ fromptr,
-                                                                 const int64_t* __restrict__ offsets,
-                                                                 int64_t outlength) {
+template <typename T, typename S>
+struct SumMinArgminOp {
+  using In = T;
+  using Out = MultiOut<T, S>;
+  struct Part {
+    S sum;
+    ValIdx<T> m;
+  };
+  T identity;  // the min identity
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t gb) const {
+    S acc = S(0);
+    T best = identity;
+    int64_t arg = -1;
+    T arg_val = T(0);
+    for (int64_t i = b; i < e; i++) {
+      const T v = p[i];
+      acc = wrap_add<S>(acc, (S)v);
+      best = (v < best) ? v : best;
+      if (i == b) {
+        arg = i;
+        arg_val = v;
+      } else if (v < arg_val) {
+        arg_val = v;
+        arg = i;
+      }
+    }
+    return Out{acc, best, arg < 0 ? -1 : gb + (arg - b)};
+  }
+  __device__ Part init() const { return Part{S(0), ValIdx<T>{identity, -1}}; }
+  __device__ Part elem(In v, int64_t gi) const { return Part{(S)v, ValIdx<T>{v, is_nan(v) ? -1 : gi}}; }
+  __device__ Part combine(Part a, Part b) const { return Part{wrap_add<S>(a.sum, b.sum), better<T, false>(a.m, b.m)}; }
+  __device__ Out finalize(Part p, const In* g, int64_t gb, int64_t ge) const {
+    T mn = identity;
+    if (p.m.idx >= 0 && p.m.val < identity) mn = p.m.val;
+    int64_t arg = -1;
+    if (gb < ge) arg = is_nan(g[gb]) ? gb : p.m.idx;
+    return Out{p.sum, mn, arg};
+  }
+  __device__ Part shfl_xor(Part p, int m) const { return Part{shfl_xor_any(p.sum, m), shfl_xor_vi(p.m, m)}; }
+};
+
+// ---- the tile -----------------------------------------------------------------
+// Epilogues own the stores: store(bin, value, gb, ge) with the list's global
+// element range [gb, ge).  PlainEpilogue writes the value; the fused
+// host-layer variants also make argmin/argmax local and write the empty-list
+// mask; the batch runner's epilogue scatters several results.
+template <typename Out>
+struct PlainEpilogue {
+  Out* toptr;
+  __device__ void store(int64_t bin, Out v, int64_t, int64_t) const { toptr[bin] = v; }
+};
+
+template <class Op, int BLOCK, int LPT, int CAP_BYTES>
+struct ReduceSmem {
+  using In = typename Op::In;
+  using Part = typename Op::Part;
+  static constexpr int CAP = CAP_BYTES / (int)sizeof(In);
+  static constexpr int NW = BLOCK / kWave;
+  static constexpr int L = BLOCK * LPT;
+  In data[CAP];
+  int64_t off[L + 1];
+  uint64_t lng[LPT][NW];
+  Part part[NW];
+};
+
+// One tile: the lists [L0, L0 + BLOCK * LPT) of `offsets` (clipped to
+// outlength); thread t owns lists t + j * BLOCK.  Ends with a barrier, so a
+// persistent caller may reuse `sm` for its next tile at once.
+template <class Op, int BLOCK, int LPT, int CAP_BYTES, class Epi>
+__device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename Op::In* __restrict__ fromptr,
+                                   const int64_t* __restrict__ offsets, int64_t outlength, int64_t L0,
+                                   ReduceSmem<Op, BLOCK, LPT, CAP_BYTES>& sm) {
   using In = typename Op::In;
   using Out = typename Op::Out;
   using Part = typename Op::Part;
   constexpr int CAP = CAP_BYTES / (int)sizeof(In);
   constexpr int NW = BLOCK / kWave;
-  constexpr int L = BLOCK * LPT;  // lists per workgroup; thread t owns lists t + j * BLOCK
-  __shared__ In s_data[CAP];
-  __shared__ int64_t s_off[L + 1];
-  __shared__ uint64_t s_long[LPT][NW];
-  __shared__ Part s_part[NW];
+  constexpr int L = BLOCK * LPT;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-  const int64_t L0 = (int64_t)blockIdx.x * L;
   const int nl = (int)((outlength - L0) < L ? (outlength - L0) : L);
 
   {
@@ -236,12 +299,12 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
 #pragma unroll
     for (int j = 0; j <= LPT; j++) {
       const int t = tid + j * BLOCK;
-      if (t <= nl) s_off[t] = r[j];
+      if (t <= nl) sm.off[t] = r[j];
     }
   }
   __syncthreads();
-  const int64_t E0 = s_off[0];
-  const int64_t span = s_off[nl] - E0;
+  const int64_t E0 = sm.off[0];
+  const int64_t span = sm.off[nl] - E0;
 
   if (span <= CAP) {
     // light tile: stage the span with coalesced loads, then serial loops
@@ -256,7 +319,7 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int64_t k = b + (int64_t)u * BLOCK + tid;
-        if (k < span) s_data[k] = r[u];
+        if (k < span) sm.data[k] = r[u];
       }
     }
     __syncthreads();
@@ -264,11 +327,11 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
     for (int j = 0; j < LPT; j++) {
       const int t = tid + j * BLOCK;
       if (t < nl) {
-        const int64_t gb = s_off[t], ge = s_off[t + 1];
-        const Out v = op.sequential(s_data, gb - E0, ge - E0, gb);
-        toptr[L0 + t] = epi.apply(v, L0 + t, gb, ge);
+        const int64_t gb = sm.off[t], ge = sm.off[t + 1];
+        epi.store(L0 + t, op.sequential(sm.data, gb - E0, ge - E0, gb), gb, ge);
       }
     }
+    __syncthreads();
     return;
   }
 
@@ -278,9 +341,9 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
     const int t = tid + j * BLOCK;
     int64_t gb = 0, ge = 0;
     if (t < nl) {
-      gb = s_off[t];
-      ge = s_off[t + 1];
-      if (ge - gb <= kShortLen) toptr[L0 + t] = epi.apply(op.sequential(fromptr, gb, ge, gb), L0 + t, gb, ge);
+      gb = sm.off[t];
+      ge = sm.off[t + 1];
+      if (ge - gb <= kShortLen) epi.store(L0 + t, op.sequential(fromptr, gb, ge, gb), gb, ge);
     }
     const int64_t len = ge - gb;
     // medium lists: the owning wave reduces each one cooperatively
@@ -289,74 +352,90 @@ __global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
       const int b = __builtin_ctzll(medium);
       medium &= medium - 1;
       const int owner = j * BLOCK + wave * kWave + b;
-      const int64_t lb = s_off[owner], le = s_off[owner + 1];
+      const int64_t lb = sm.off[owner], le = sm.off[owner + 1];
       Part p = op.init();
       for (int64_t i = lb + lane; i < le; i += kWave) p = op.combine(p, op.elem(fromptr[i], i));
 #pragma unroll
       for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
-      if (lane == 0) toptr[L0 + owner] = epi.apply(op.finalize(p, fromptr, lb, le), L0 + owner, lb, le);
+      if (lane == 0) epi.store(L0 + owner, op.finalize(p, fromptr, lb, le), lb, le);
     }
     const uint64_t lng = __ballot(t < nl && len > kWaveLen);
-    if (lane == 0) s_long[j][wave] = lng;
+    if (lane == 0) sm.lng[j][wave] = lng;
   }
   __syncthreads();
   // long lists: the whole workgroup
   for (int j = 0; j < LPT; j++) {
     for (int w = 0; w < NW; w++) {
-      uint64_t m = s_long[j][w];
+      uint64_t m = sm.lng[j][w];
       while (m) {
         const int b = __builtin_ctzll(m);
         m &= m - 1;
         const int owner = j * BLOCK + w * kWave + b;
-        const int64_t lb = s_off[owner], le = s_off[owner + 1];
+        const int64_t lb = sm.off[owner], le = sm.off[owner + 1];
         Part p = op.init();
 #pragma unroll 4
         for (int64_t i = lb + tid; i < le; i += BLOCK) p = op.combine(p, op.elem(fromptr[i], i));
 #pragma unroll
         for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
-        if (lane == 0) s_part[wave] = p;
+        if (lane == 0) sm.part[wave] = p;
         __syncthreads();
         if (tid == 0) {
-          Part q = s_part[0];
-          for (int k = 1; k < NW; k++) q = op.combine(q, s_part[k]);
-          toptr[L0 + owner] = epi.apply(op.finalize(q, fromptr, lb, le), L0 + owner, lb, le);
+          Part q = sm.part[0];
+          for (int k = 1; k < NW; k++) q = op.combine(q, sm.part[k]);
+          epi.store(L0 + owner, op.finalize(q, fromptr, lb, le), lb, le);
         }
         __syncthreads();
       }
     }
   }
+  __syncthreads();
+  (void)sizeof(Out);
 }
+
+template <class Op, int BLOCK, int LPT, int CAP_BYTES, class Epi>
+__global__ __launch_bounds__(BLOCK) void segmented_reduce_kernel(Op op, Epi epi,
+                                                                 const typename Op::In* __restrict__ fromptr,
+                                                                 const int64_t* __restrict__ offsets,
+                                                                 int64_t outlength) {
+  __shared__ ReduceSmem<Op, BLOCK, LPT, CAP_BYTES> sm;
+  reduce_tile<Op, BLOCK, LPT, CAP_BYTES, Epi>(op, epi, fromptr, offsets, outlength, (int64_t)blockIdx.x * BLOCK * LPT,
+                                             sm);
+}
+
+}  // namespace akhip
+
+namespace akhip {
 
 // Fused epilogue of the host layer's local (axis=-1) arg-reductions: the
 // global index becomes local (awkward_NumpyArray_reduce_adjust_starts_64 with
 // starts = offsets[:-1]) and the empty-list mask is written in the same pass
 // (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64).
 struct LocalArgEpilogue {
+  int64_t* toptr;
   int8_t* mask;  // may be null
-  __device__ int64_t apply(int64_t v, int64_t bin, int64_t gb, int64_t ge) const {
+  __device__ void store(int64_t bin, int64_t v, int64_t gb, int64_t ge) const {
     if (mask) mask[bin] = (ge - gb > 0) ? 0 : 1;
-    return v >= 0 ? v - gb : v;
+    toptr[bin] = v >= 0 ? v - gb : v;
   }
 };
 
 // Fused mask for value reductions (max/min with mask_identity=True).
+template <typename Out>
 struct MaskEpilogue {
+  Out* toptr;
   int8_t* mask;
-  template <typename Out>
-  __device__ Out apply(Out v, int64_t bin, int64_t gb, int64_t ge) const {
+  __device__ void store(int64_t bin, Out v, int64_t gb, int64_t ge) const {
     if (mask) mask[bin] = (ge - gb > 0) ? 0 : 1;
-    return v;
+    toptr[bin] = v;
   }
 };
 
-
 template <class Op, int BLOCK, int LPT, int CAP, class Epi>
-inline Error launch_reduce_v(const Op& op, const Epi& epi, typename Op::Out* toptr, const typename Op::In* fromptr,
-                             const int64_t* offsets, int64_t outlength) {
+inline Error launch_reduce_v(const Op& op, const Epi& epi, const typename Op::In* fromptr, const int64_t* offsets,
+                             int64_t outlength) {
   if (outlength <= 0) return success();
   const unsigned grid = (unsigned)ceil_div(outlength, (int64_t)BLOCK * LPT);
-  segmented_reduce_kernel<Op, BLOCK, LPT, CAP, Epi>
-      <<<grid, BLOCK, 0, stream()>>>(op, epi, toptr, fromptr, offsets, outlength);
+  segmented_reduce_kernel<Op, BLOCK, LPT, CAP, Epi><<<grid, BLOCK, 0, stream()>>>(op, epi, fromptr, offsets, outlength);
   AKHIP_LAUNCH_CHECK("segmented_reduce_kernel");
   return success();
 }
@@ -377,23 +456,24 @@ constexpr int kVariants = 10;
 template <class Op>
 inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
                            const int64_t* offsets, int64_t outlength) {
-  return launch_reduce_v<Op, 128, 1, 16384>(op, PlainEpilogue{}, toptr, fromptr, offsets, outlength);
+  using E = PlainEpilogue<typename Op::Out>;
+  return launch_reduce_v<Op, 128, 1, 16384, E>(op, E{toptr}, fromptr, offsets, outlength);
 }
 
 template <class Op, class Epi>
-inline Error launch_reduce(const Op& op, typename Op::Out* toptr, const typename Op::In* fromptr,
-                           const int64_t* offsets, int64_t outlength, int variant, Epi epi) {
+inline Error launch_reduce(const Op& op, const typename Op::In* fromptr, const int64_t* offsets, int64_t outlength,
+                           int variant, Epi epi) {
   switch (variant) {
-    case 1: return launch_reduce_v<Op, 256, 4, 8192>(op, epi, toptr, fromptr, offsets, outlength);
-    case 2: return launch_reduce_v<Op, 256, 2, 16384>(op, epi, toptr, fromptr, offsets, outlength);
-    case 3: return launch_reduce_v<Op, 128, 1, 16384>(op, epi, toptr, fromptr, offsets, outlength);
-    case 4: return launch_reduce_v<Op, 256, 1, 65536>(op, epi, toptr, fromptr, offsets, outlength);
-    case 5: return launch_reduce_v<Op, 256, 8, 8192>(op, epi, toptr, fromptr, offsets, outlength);
-    case 6: return launch_reduce_v<Op, 64, 1, 8192>(op, epi, toptr, fromptr, offsets, outlength);
-    case 7: return launch_reduce_v<Op, 128, 2, 16384>(op, epi, toptr, fromptr, offsets, outlength);
-    case 8: return launch_reduce_v<Op, 64, 4, 8192>(op, epi, toptr, fromptr, offsets, outlength);
-    case 9: return launch_reduce_v<Op, 128, 1, 8192>(op, epi, toptr, fromptr, offsets, outlength);
-    default: return launch_reduce_v<Op, 256, 1, 32768>(op, epi, toptr, fromptr, offsets, outlength);
+    case 1: return launch_reduce_v<Op, 256, 4, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    case 2: return launch_reduce_v<Op, 256, 2, 16384, Epi>(op, epi, fromptr, offsets, outlength);
+    case 3: return launch_reduce_v<Op, 128, 1, 16384, Epi>(op, epi, fromptr, offsets, outlength);
+    case 4: return launch_reduce_v<Op, 256, 1, 65536, Epi>(op, epi, fromptr, offsets, outlength);
+    case 5: return launch_reduce_v<Op, 256, 8, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    case 6: return launch_reduce_v<Op, 64, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    case 7: return launch_reduce_v<Op, 128, 2, 16384, Epi>(op, epi, fromptr, offsets, outlength);
+    case 8: return launch_reduce_v<Op, 64, 4, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    case 9: return launch_reduce_v<Op, 128, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    default: return launch_reduce_v<Op, 256, 1, 32768, Epi>(op, epi, fromptr, offsets, outlength);
   }
 }
 

@@ -16,8 +16,8 @@ extern "C" {
   }                                                                                                        \
   Error awkward_hip_reduce_argmax_local_##TN(int64_t* toptr, int8_t* mask, const T* fromptr,                 \
                                           const int64_t* offsets, int64_t outlength, int variant) {        \
-    return launch_reduce(ArgOp<T, true>{}, toptr, fromptr, offsets, outlength, variant,         \
-                         LocalArgEpilogue{mask});                                                          \
+    return launch_reduce(ArgOp<T, true>{}, fromptr, offsets, outlength, variant, \
+                         LocalArgEpilogue{toptr, mask});                                                          \
   }
 ARG(int8, int8_t)
 ARG(uint8, uint8_t)

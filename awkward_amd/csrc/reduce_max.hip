@@ -17,8 +17,8 @@ extern "C" {
   }                                                                                                        \
   Error awkward_hip_reduce_max_masked_##TN(T* toptr, int8_t* mask, const T* fromptr, const int64_t* offsets, \
                                            int64_t outlength, T identity, int variant) {                   \
-    return launch_reduce(ExtremumOp<T, true>{identity}, toptr, fromptr, offsets, outlength, variant, \
-                         MaskEpilogue{mask});                                                              \
+    return launch_reduce(ExtremumOp<T, true>{identity}, fromptr, offsets, outlength, variant, \
+                         MaskEpilogue<T>{toptr, mask});                                                              \
   }
 EXT(int8, int8_t)
 EXT(uint8, uint8_t)

@@ -16,7 +16,7 @@ extern "C" {
   }                                                                                                      \
   Error awkward_hip_reduce_sum_##OUTN##_##INN(OUT* toptr, const IN* fromptr, const int64_t* offsets,     \
                                               int64_t outlength, int variant) {                          \
-    return launch_reduce(SumOp<OUT, IN>{}, toptr, fromptr, offsets, outlength, variant, PlainEpilogue{}); \
+    return launch_reduce(SumOp<OUT, IN>{}, fromptr, offsets, outlength, variant, PlainEpilogue<OUT>{toptr}); \
   }
 SUM(int64, int8, int64_t, int8_t)
 SUM(uint64, uint8, uint64_t, uint8_t)
@@ -51,7 +51,7 @@ SUM(float64, float32, double, float)
   }                                                                                                      \
   Error awkward_hip_reduce_sum_##OUTN##_bool(OUT* toptr, const bool* fromptr, const int64_t* offsets,    \
                                              int64_t outlength, int variant) {                           \
-    return launch_reduce(SumBoolOp<OUT>{}, toptr, fromptr, offsets, outlength, variant, PlainEpilogue{}); \
+    return launch_reduce(SumBoolOp<OUT>{}, fromptr, offsets, outlength, variant, PlainEpilogue<OUT>{toptr}); \
   }
 SUMBOOL(int64, int64_t)
 SUMBOOL(int32, int32_t)

@@ -147,6 +147,49 @@ class PairMax:
         return o
 
 
+class Combinations:
+    """ak.combinations(ev, n)'s carries without a host round trip per call:
+    the tuple count is read once at construction to size the outputs, then
+    run() enqueues combinations_length + the decode (capturable)."""
+
+    def __init__(self, events, n, replacement=False):
+        layout = events.layout
+        self.offsets = layout.offsets
+        self.length = layout.length
+        self.n, self.replacement = n, replacement
+        dev = self.offsets.device
+        self.tooffsets = torch.empty(self.length + 1, dtype=torch.int64, device=dev)
+        self.totallen = torch.empty(1, dtype=torch.int64, device=dev)
+        self.starts = self.offsets[:-1]
+        self.stops = self.offsets[1:]
+        P, I64 = ctypes.c_void_p, ctypes.c_int64
+        self._len = _fn("awkward_ListArray64_combinations_length_64", [P, P, I64, ctypes.c_bool, P, P, I64])
+        self._dec = _fn("awkward_hip_combinations_from_offsets", [P, P, P, P, P, I64, I64, ctypes.c_bool, I64])
+        self._run_length()
+        torch.cuda.synchronize()
+        self.total = int(self.totallen.item())
+        self.carry = [torch.empty(max(self.total, 1), dtype=torch.int64, device=dev) for _ in range(n)]
+        from awkward_amd._kernels import PointerArray
+
+        self.table = PointerArray(self.carry)
+        self.toindex = torch.empty(n, dtype=torch.int64, device=dev)
+
+    def _run_length(self):
+        set_stream()
+        _check(self._len(self.totallen.data_ptr(), self.tooffsets.data_ptr(), self.n, self.replacement,
+                         self.starts.data_ptr(), self.stops.data_ptr(), self.length))
+
+    def run(self):
+        self._run_length()
+        _check(self._dec(self.table.data_ptr(), self.toindex.data_ptr(), self.tooffsets.data_ptr(),
+                         self.starts.data_ptr(), self.stops.data_ptr(), self.length, self.n, self.replacement,
+                         self.total))
+        return self.carry
+
+    def algorithmic_bytes(self):
+        return 16 * self.length + 8 * (self.length + 1) + 8 * self.n * self.total
+
+
 def pair_expr(p0, p1):
     """The pair expression over gathered legs (float32 device tensors)."""
     out = torch.empty_like(p0[0])

@@ -132,6 +132,123 @@ def run_config1(ak, backend, torch, reps=5):
     return result
 
 
+def config3_step(ak, np, ev):
+    pairs = ak.combinations(ev, 2)
+    p0, p1 = pairs["0"], pairs["1"]
+    expr = 2 * p0.pt * p1.pt * (np.cosh(p0.eta - p1.eta) - np.cos(p0.phi - p1.phi))
+    triples = ak.combinations(ev, 3)
+    return ak.max(expr, axis=-1), ak.argmax(expr, axis=-1), triples
+
+
+def run_config3(ak, backend, torch, n_events=20_000_000, steps=5):
+    """Combinations n=2 and n=3 on 20M events (Poisson(4) clipped [0, 30],
+    seed 3), the float32 pair expression, max/argmax over pairs per event."""
+    import numpy as np
+
+    from awkward_amd.fused import Combinations, Graph, PairMax
+
+    off, fields = make_events(n_events, seed=3, lam=4, clip=30)
+    fields = {k: fields[k] for k in ("pt", "eta", "phi")}
+    ev = ak.from_offsets(off, fields, backend)
+    torch.cuda.synchronize()
+    out = {"events": n_events, "particles": int(off[-1])}
+    out["plain"] = time_form(torch, lambda: config3_step(ak, np, ev), steps, 1, n_events)
+    pm = PairMax(ev)
+    tri = Combinations(ev, 3)
+    out["pairs"] = int(ak.num(ak.combinations(ev, 2)).layout.data.sum().item())
+    out["triples"] = tri.total
+
+    def fused():
+        return pm.run(), tri.run()
+
+    graph = Graph(fused)
+    out["fused_graph"] = time_form(torch, graph.replay, steps, 1, n_events)
+    out["speedup"] = round(out["fused_graph"]["events_per_s"] / out["plain"]["events_per_s"], 2)
+    # the triples' decode alone: write-bound, 8 * 3 * P bytes
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tri.run()
+    s.record()
+    tri.run()
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e)
+    out["combinations3"] = {"ms": round(ms, 4), "GB/s": round(tri.algorithmic_bytes() / ms / 1e6, 1),
+                            "bytes": tri.algorithmic_bytes()}
+    out["what"] = ("plain: ak.combinations(ev,2) + torch ufunc expression + ak.max/argmax + ak.combinations(ev,3); "
+                   "fused_graph: awkward_hip_pair_max + combinations_length/decode for n=3 in one hipGraph")
+    return out
+
+
+def make_batch(seed=4, narrays=256, min_exp=3.0, max_exp=5.0, zipf_cap=100_000):
+    """The config-4 generator (seed 4): per array a list count 10**U(min_exp,
+    max_exp), lengths Poisson(2) / Poisson(50) / Zipf(1.5) capped, 5% of the
+    arrays all-empty, dtype int32 / int64 / float32 / float64, values
+    integers(-1000, 1000) or standard normal."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(narrays):
+        n = int(10 ** rng.uniform(min_exp, max_exp))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            counts = rng.poisson(2, n)
+        elif kind == 1:
+            counts = rng.poisson(50, n)
+        else:
+            counts = np.minimum(rng.zipf(1.5, n), zipf_cap)
+        if rng.random() < 0.05:
+            counts = np.zeros(n, dtype=np.int64)
+        off = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(counts, out=off[1:])
+        dt = [np.int32, np.int64, np.float32, np.float64][int(rng.integers(0, 4))]
+        N = int(off[-1])
+        x = rng.integers(-1000, 1000, N).astype(dt) if np.dtype(dt).kind == "i" else rng.standard_normal(N).astype(dt)
+        out.append((off, x))
+    return out
+
+
+def run_config4(ak, torch, steps=3):
+    """Batched uneven workload: per array sum, min, argmin and argsort.
+    plain = the registry path array by array; batched = the persistent
+    work-queue kernels (autotuned tiles) replayed from one hipGraph."""
+    import numpy as np
+
+    from awkward_amd.backend import HipBackend
+    from awkward_amd.batch import BatchRunner
+    from awkward_amd.fused import Graph
+
+    batch = make_batch()
+    n_lists = int(sum(len(o) - 1 for o, _ in batch))
+    n_elem = int(sum(len(x) for _, x in batch))
+    be = HipBackend(fused=False)
+    arrays = [ak.from_offsets(o, x, be) for o, x in batch]
+    torch.cuda.synchronize()
+
+    def plain():
+        for a in arrays:
+            ak.sum(a, axis=-1)
+            ak.min(a, axis=-1)
+            ak.argmin(a, axis=-1)
+            ak.argsort(a, axis=-1)
+
+    out = {"arrays": len(batch), "lists": n_lists, "elements": n_elem,
+           "sample": "list counts log-uniform in [1e3, 1e5] (the configuration's [1e3, 1e7] would hold ~2.6e10 "
+                     "elements); every other parameter as configured, seed 4"}
+    out["plain"] = time_form(torch, plain, steps, 1, n_lists)
+    dev = [(a.layout.offsets, a.layout.content.data) for a in arrays]
+    runner = BatchRunner(dev)
+    out["autotune"] = runner.tune(reps=3)
+    graph = Graph(runner.run)
+    out["batched_graph"] = time_form(torch, graph.replay, steps * 3, 1, n_lists)
+    for form in ("plain", "batched_graph"):
+        out[form]["lists_per_s"] = out[form].pop("events_per_s")
+    out["speedup"] = round(out["batched_graph"]["lists_per_s"] / out["plain"]["lists_per_s"], 2)
+    ms = out["batched_graph"]["ms_per_step"]
+    out["batched_GB/s"] = round(runner.algorithmic_bytes() / ms / 1e6, 1)
+    return out
+
+
 def cpu_baseline(ak, off, fields, sample_events):
     import numpy as np
 
@@ -178,6 +295,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--no-config4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each op is independent per "
@@ -240,6 +359,15 @@ def main():
                 "alg_bytes": alg_bytes}
 
     config1 = None if args.no_config1 else run_config1(ak, tuned_be, torch)
+    del ev, ev_plain, pipe, graph
+    torch.cuda.empty_cache()
+    config3 = None if args.no_config3 else run_config3(ak, tuned_be, torch)
+    if config3:
+        log(f"config3: plain {config3['plain']['events_per_s']:.4g} ev/s, fused {config3['fused_graph']['events_per_s']:.4g} ev/s")
+    torch.cuda.empty_cache()
+    config4 = None if args.no_config4 else run_config4(ak, torch)
+    if config4:
+        log(f"config4: plain {config4['plain']['lists_per_s']:.4g} lists/s, batched {config4['batched_graph']['lists_per_s']:.4g} lists/s")
     cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
 
     line = {
@@ -263,6 +391,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config1": config1,
+        "config3": config3,
+        "config4": config4,
     }
     print(json.dumps(line), flush=True)
 

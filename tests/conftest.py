@@ -10,7 +10,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-_GPU_ORDER = ["test_hip_golden", "test_hip_parity", "test_pipeline", "test_fused"]
+_GPU_ORDER = ["test_hip_golden", "test_hip_parity", "test_pipeline", "test_fused", "test_batch"]
 
 
 def pytest_collection_modifyitems(session, config, items):

@@ -109,3 +109,26 @@ def test_graph_replays_match_eager(hip):
         for want, got in zip(eager, res):
             for k in want:
                 assert same_bits(want[k].cpu().numpy(), got[k].cpu().numpy()), k
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_combinations_runner_matches_registry_path(hip, n):
+    from awkward_amd.fused import Combinations, Graph
+
+    off, fields = events(9, 20000, 4, 30)
+    ev = ak.from_offsets(off, fields, hip)
+    want = ak.combinations(ev, n)
+    runner = Combinations(ev, n)
+    got = runner.run()
+    wl = want.layout
+    assert same_bits(wl.offsets.cpu().numpy(), runner.tooffsets.cpu().numpy())
+    for k in range(n):
+        idx = wl.content.contents[k].index
+        assert same_bits(idx.cpu().numpy(), got[k][: runner.total].cpu().numpy())
+    g = Graph(runner.run)
+    for _ in range(3):
+        for c in runner.carry:
+            c.fill_(-1)
+        g.replay()
+        for k in range(n):
+            assert same_bits(wl.content.contents[k].index.cpu().numpy(), runner.carry[k][: runner.total].cpu().numpy())
