@@ -13,8 +13,12 @@ The per-array results match the registry path on the same array:
 ``ak.sum``, ``ak.min`` / ``ak.argmin`` (mask_identity=True: value + empty
 mask) and ``ak.argsort`` (stable, ascending).
 
-``variant`` selects the reduce tile per dtype (see ``REDUCE_VARIANTS``); the
-autotuner (``tune()``) times them on the batch itself and keeps the fastest.
+``variants`` selects the reduce tile per dtype (the ten tiles of
+``dispatch.VARIANTS``); the autotuner (``tune()``) times them on the batch
+itself and keeps the fastest.  ``variants="learned"`` instead enqueues, per
+dtype group, the dispatcher's statistics pass over all the group's offsets,
+the MLP inference and the ten tile launches predicated on its choice
+(``awkward_hip_batch_reduce_<T>_chosen``) -- still no host synchronisation.
 """
 
 from __future__ import annotations
@@ -24,11 +28,12 @@ import ctypes
 import numpy as np
 import torch
 
+from awkward_amd import dispatch
 from awkward_amd._kernels import KernelError, set_stream
 from awkward_amd._lib import lib
 
 #: (threads, lists per thread, LDS bytes) of awkward_hip_batch_reduce_* variants
-REDUCE_VARIANTS = [(128, 1, 16384), (64, 1, 8192), (256, 4, 8192), (256, 8, 8192), (256, 1, 32768)]
+REDUCE_VARIANTS = dispatch.VARIANTS
 
 _SUM = {np.dtype(np.int32): torch.int64, np.dtype(np.int64): torch.int64,
         np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
@@ -56,56 +61,88 @@ def min_identity(dtype: np.dtype):
 
 
 class _Group:
-    def __init__(self, dtype, members, arrays, device, variant):
+    def __init__(self, dtype, members, arrays, device, variant, dispatcher=None):
         self.dtype = dtype
         self.name = _NAME[dtype]
         self.members = members
         self.variant = variant
+        self.dispatcher = dispatcher
         self.counter = torch.zeros(2, dtype=torch.int64, device=device)
         self.outputs = {}
         self._build(arrays, device)
+
+    def _reduce_table(self, arrays, L, device):
+        rows, tile = [], 0
+        for i in self.members:
+            offsets, content = arrays[i]
+            n = offsets.shape[0] - 1
+            out = self.outputs[i]
+            rows.append([content.data_ptr(), offsets.data_ptr(), out["sum"].data_ptr(), out["min"].data_ptr(),
+                         out["argmin"].data_ptr(), out["mask"].data_ptr(), n, tile])
+            tile += -(-n // L) if n > 0 else 0
+        return torch.from_numpy(np.array(rows, dtype=np.uint64).view(np.int64)).to(device), tile
 
     def _build(self, arrays, device):
         lib_ = lib()
         tile_lists = lib_.awkward_hip_batch_reduce_tile_lists
         tile_lists.argtypes = [ctypes.c_int]
         tile_lists.restype = ctypes.c_int64
-        L = int(tile_lists(self.variant))
         sort_lists = int(lib_.awkward_hip_batch_sort_tile_lists())
-        red_rows, sort_rows = [], []
-        rtile = stile = elem = 0
+        sort_rows = []
+        stile = elem = 0
         for i in self.members:
             offsets, content = arrays[i]
             n = offsets.shape[0] - 1
-            out = {
-                "sum": torch.empty(n, dtype=_SUM[self.dtype], device=device),
-                "min": torch.empty(n, dtype=content.dtype, device=device),
-                "argmin": torch.empty(n, dtype=torch.int64, device=device),
-                "mask": torch.empty(n, dtype=torch.int8, device=device),
-                "argsort": torch.empty(max(content.shape[0], 1), dtype=torch.int64, device=device),
-            }
-            self.outputs[i] = out
-            red_rows.append([content.data_ptr(), offsets.data_ptr(), out["sum"].data_ptr(), out["min"].data_ptr(),
-                             out["argmin"].data_ptr(), out["mask"].data_ptr(), n, rtile])
+            if i not in self.outputs:
+                self.outputs[i] = {
+                    "sum": torch.empty(n, dtype=_SUM[self.dtype], device=device),
+                    "min": torch.empty(n, dtype=content.dtype, device=device),
+                    "argmin": torch.empty(n, dtype=torch.int64, device=device),
+                    "mask": torch.empty(n, dtype=torch.int8, device=device),
+                    "argsort": torch.empty(max(content.shape[0], 1), dtype=torch.int64, device=device),
+                }
+            out = self.outputs[i]
             sort_rows.append([content.data_ptr(), offsets.data_ptr(), out["argsort"].data_ptr(), n,
                               content.shape[0], stile, elem])
-            rtile += -(-n // L) if n > 0 else 0
             stile += -(-n // sort_lists) if n > 0 else 0
             elem += content.shape[0]
-        self.reduce_tiles, self.sort_tiles, self.elements = rtile, stile, elem
-        self.red_table = torch.from_numpy(np.array(red_rows, dtype=np.uint64).view(np.int64)).to(device)
+        self.sort_tiles, self.elements = stile, elem
         self.sort_table = torch.from_numpy(np.array(sort_rows, dtype=np.uint64).view(np.int64)).to(device)
         P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         self._reset = _fn("awkward_hip_batch_reset", [P])
         self._reduce = _fn(f"awkward_hip_batch_reduce_{self.name}", [P, I32, I64, P, _CT[self.name], I32])
         self._sort = _fn(f"awkward_hip_batch_argsort_{self.name}", [P, I32, I64, P, ctypes.c_bool, I64])
         self.identity = min_identity(self.dtype)
+        if self.dispatcher is None:
+            self.red_table, self.reduce_tiles = self._reduce_table(arrays, int(tile_lists(self.variant)), device)
+            return
+        # learned: a table per tile shape, the statistics plan of the whole group
+        self._chosen = _fn(f"awkward_hip_batch_reduce_{self.name}_chosen", [P, P, I32, P, _CT[self.name], P])
+        self.var_tables = [self._reduce_table(arrays, int(tile_lists(v)), device) for v in range(len(REDUCE_VARIANTS))]
+        self._ptrs = (ctypes.c_void_p * len(REDUCE_VARIANTS))(*[t.data_ptr() for t, _ in self.var_tables])
+        self._tiles = (ctypes.c_int64 * len(REDUCE_VARIANTS))(*[n for _, n in self.var_tables])
+        entries = [(arrays[i][0], arrays[i][0].shape[0] - 1, 0) for i in self.members]
+        sizes = [(sum(arrays[i][0].shape[0] - 1 for i in self.members),
+                  sum(arrays[i][1].shape[0] for i in self.members))]
+        from awkward_amd.learned import OP_CODE
+
+        self.plan, self.meta, self.choice = self.dispatcher.choose_device(
+            OP_CODE["batch"], entries, sizes, [self.dtype.itemsize], device)
 
     def run_reduce(self):
         n = len(self.members)
         _check(self._reset(self.counter.data_ptr()))
-        _check(self._reduce(self.red_table.data_ptr(), n, self.reduce_tiles, self.counter.data_ptr(),
-                            self.identity, self.variant))
+        if self.dispatcher is None:
+            _check(self._reduce(self.red_table.data_ptr(), n, self.reduce_tiles, self.counter.data_ptr(),
+                                self.identity, self.variant))
+            return
+        self.dispatcher.run_device(self.plan, self.meta, self.choice)
+        _check(self._chosen(ctypes.addressof(self._ptrs), ctypes.addressof(self._tiles), n, self.counter.data_ptr(),
+                            self.identity, self.choice.data_ptr()))
+
+    def chosen(self):
+        """The learned tile (reads the device choice: synchronises)."""
+        return int(self.choice[0].item()) if self.dispatcher is not None else self.variant
 
     def run_sort(self):
         n = len(self.members)
@@ -115,16 +152,26 @@ class _Group:
 
 
 class BatchRunner:
-    def __init__(self, arrays, variants=None, argsort=True):
-        """arrays: list of (offsets, content) torch tensors on one device."""
+    def __init__(self, arrays, variants=None, argsort=True, dispatcher=None):
+        """arrays: list of (offsets, content) torch tensors on one device;
+        variants: {dtype name: tile} or "learned" (then `dispatcher`, or the
+        default trained one, chooses on the device)."""
         self.arrays = arrays
         self.argsort = argsort
         device = arrays[0][0].device
         groups = {}
         for i, (offsets, content) in enumerate(arrays):
             groups.setdefault(_TORCH2NP[content.dtype], []).append(i)
+        if variants == "learned":
+            if dispatcher is None:
+                from awkward_amd.learned import Dispatcher
+
+                dispatcher = Dispatcher()
+            variants = {}
+        else:
+            dispatcher = None
         variants = variants or {}
-        self.groups = [_Group(dt, members, arrays, device, variants.get(_NAME[dt], 0))
+        self.groups = [_Group(dt, members, arrays, device, variants.get(_NAME[dt], 0), dispatcher)
                        for dt, members in sorted(groups.items(), key=lambda kv: kv[0].name)]
         # sort scratch sized before any capture: the largest group
         need = max(g.elements for g in self.groups)
@@ -167,6 +214,8 @@ class BatchRunner:
         fastest (the autotuner's choice) and return the timings."""
         timings = {}
         for g in self.groups:
+            if g.dispatcher is not None:
+                raise ValueError("tune() times fixed tiles; this runner chooses them with the learned dispatcher")
             best, best_ms = g.variant, None
             times = {}
             for v in range(len(REDUCE_VARIANTS)):

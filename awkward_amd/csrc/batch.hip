@@ -12,7 +12,9 @@
 //   awkward_hip_batch_reduce_<T>: per list, sum (awkward_reduce_sum's output
 //     type), min (awkward_reduce_min with its identity) and the local argmin
 //     with the empty-list mask (awkward_reduce_argmin + adjust_starts +
-//     reduce_mask), all from one pass over the elements;
+//     reduce_mask), all from one pass over the elements, with one of the ten
+//     reduce tiles; awkward_hip_batch_reduce_<T>_chosen enqueues all ten,
+//     predicated on a tile chosen on the device (learned.hip);
 //   awkward_hip_batch_argsort_<T>: per list, the stable argsort
 //     (awkward_argsort), local indices.
 //
@@ -79,11 +81,15 @@ __global__ void reset_counter(unsigned long long* counter) { *counter = 0ull; }
 template <typename T, typename S, int BLOCK, int LPT, int CAP>
 __global__ __launch_bounds__(BLOCK) void batch_reduce_kernel(const BatchReduceItem* __restrict__ items, int nitems,
                                                              int64_t total_tiles, unsigned long long* counter,
-                                                             T identity) {
+                                                             T identity, const int32_t* __restrict__ choice,
+                                                             int self) {
   using Op = SumMinArgminOp<T, S>;
   __shared__ ReduceSmem<Op, BLOCK, LPT, CAP> sm;
   __shared__ int64_t s_tile;
   __shared__ int s_item;
+  // a launch predicated on a device-side choice: every workgroup leaves at once
+  // unless this variant was chosen
+  if (choice != nullptr && choice[0] != self) return;
   const Op op{identity};
   for (;;) {
     if (threadIdx.x == 0) {
@@ -132,15 +138,33 @@ __global__ __launch_bounds__(sortimpl::kBlock) void batch_argsort_kernel(const B
 
 template <typename T, typename S, int BLOCK, int LPT, int CAP>
 Error launch_batch_reduce(const BatchReduceItem* items, int nitems, int64_t total_tiles, unsigned long long* counter,
-                          T identity) {
+                          T identity, const int32_t* choice, int self) {
   // persistent grid: a few workgroups per CU, never more than the tiles
   int64_t grid = (int64_t)cu_count() * (BLOCK <= 64 ? 16 : BLOCK <= 128 ? 8 : 4);
   if (grid > total_tiles) grid = total_tiles;
   if (grid < 1) grid = 1;
-  batch_reduce_kernel<T, S, BLOCK, LPT, CAP><<<(unsigned)grid, BLOCK, 0, stream()>>>(items, nitems, total_tiles,
-                                                                                     counter, identity);
+  batch_reduce_kernel<T, S, BLOCK, LPT, CAP><<<(unsigned)grid, BLOCK, 0, stream()>>>(
+      items, nitems, total_tiles, counter, identity, choice, self);
   AKHIP_LAUNCH_CHECK("batch_reduce_kernel");
   return success();
+}
+
+// The ten tiles of launch_reduce (threads x lists per thread x LDS bytes).
+template <typename T, typename S>
+Error launch_batch_variant(int variant, const BatchReduceItem* it, int nitems, int64_t tiles,
+                           unsigned long long* counter, T identity, const int32_t* choice) {
+  switch (variant) {
+    case 1: return launch_batch_reduce<T, S, 256, 4, 8192>(it, nitems, tiles, counter, identity, choice, 1);
+    case 2: return launch_batch_reduce<T, S, 256, 2, 16384>(it, nitems, tiles, counter, identity, choice, 2);
+    case 3: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, tiles, counter, identity, choice, 3);
+    case 4: return launch_batch_reduce<T, S, 256, 1, 65536>(it, nitems, tiles, counter, identity, choice, 4);
+    case 5: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, tiles, counter, identity, choice, 5);
+    case 6: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, tiles, counter, identity, choice, 6);
+    case 7: return launch_batch_reduce<T, S, 128, 2, 16384>(it, nitems, tiles, counter, identity, choice, 7);
+    case 8: return launch_batch_reduce<T, S, 64, 4, 8192>(it, nitems, tiles, counter, identity, choice, 8);
+    case 9: return launch_batch_reduce<T, S, 128, 1, 8192>(it, nitems, tiles, counter, identity, choice, 9);
+    default: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, tiles, counter, identity, choice, 0);
+  }
 }
 
 }  // namespace
@@ -152,13 +176,8 @@ extern "C" {
 
 // Lists per tile of each batch-reduce variant (the host numbers the tiles).
 int64_t awkward_hip_batch_reduce_tile_lists(int variant) {
-  switch (variant) {
-    case 1: return 64;
-    case 2: return 1024;
-    case 3: return 2048;
-    case 4: return 256;
-    default: return 128;
-  }
+  static const int64_t lists[10] = {256, 1024, 512, 128, 256, 2048, 64, 256, 256, 128};
+  return (variant >= 0 && variant < 10) ? lists[variant] : lists[0];
 }
 
 int64_t awkward_hip_batch_sort_tile_lists(void) { return sortimpl::kBlock; }
@@ -169,18 +188,28 @@ Error awkward_hip_batch_reset(unsigned long long* counter) {
   return success();
 }
 
+// tables / total_tiles (host arrays of 10): the item table and tile count
+// built for each variant's lists per tile; choice (device): the variant to
+// run (awkward_hip_dispatch_infer's output).  All ten launches are enqueued,
+// each leaves at once unless chosen, so no host synchronisation is needed.
 #define BATCH_REDUCE(TN, T, S)                                                                                   \
   Error awkward_hip_batch_reduce_##TN(const void* items, int32_t nitems, int64_t total_tiles,                   \
                                       unsigned long long* counter, T identity, int32_t variant) {              \
-    const BatchReduceItem* it = (const BatchReduceItem*)items;                                                 \
     if (nitems <= 0 || total_tiles <= 0) return success();                                                     \
-    switch (variant) {                                                                                         \
-      case 1: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, total_tiles, counter, identity);       \
-      case 2: return launch_batch_reduce<T, S, 256, 4, 8192>(it, nitems, total_tiles, counter, identity);      \
-      case 3: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, total_tiles, counter, identity);      \
-      case 4: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, total_tiles, counter, identity);     \
-      default: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, total_tiles, counter, identity);    \
+    return launch_batch_variant<T, S>(variant, (const BatchReduceItem*)items, nitems, total_tiles, counter,    \
+                                      identity, nullptr);                                                      \
+  }                                                                                                            \
+  Error awkward_hip_batch_reduce_##TN##_chosen(const void* const* tables, const int64_t* total_tiles,          \
+                                               int32_t nitems, unsigned long long* counter, T identity,        \
+                                               const int32_t* choice) {                                        \
+    if (nitems <= 0) return success();                                                                         \
+    for (int v = 0; v < 10; v++) {                                                                             \
+      if (total_tiles[v] <= 0) continue;                                                                       \
+      Error e = launch_batch_variant<T, S>(v, (const BatchReduceItem*)tables[v], nitems, total_tiles[v],       \
+                                           counter, identity, choice);                                         \
+      if (e.str) return e;                                                                                     \
     }                                                                                                          \
+    return success();                                                                                          \
   }
 BATCH_REDUCE(int32, int32_t, int64_t)
 BATCH_REDUCE(int64, int64_t, int64_t)

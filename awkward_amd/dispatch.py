@@ -35,7 +35,8 @@ VARIANTS = [
 LDS_PER_CU = 160 * 1024
 WAVES_PER_CU = 32
 
-#: optional override: callable(op, n_lists, n_elements, itemsize) -> variant
+#: optional override: callable(op, n_lists, n_elements, itemsize, offsets) -> variant
+#: (``awkward_amd.learned.enable()`` installs the learned dispatcher here)
 chooser = None
 
 
@@ -66,9 +67,9 @@ def heuristic_variant(n_lists: int, n_elements: int, itemsize: int) -> int:
     return 4 if best is None else best  # long lists: the largest stage, heavy paths
 
 
-def choose_variant(op: str, n_lists: int, n_elements: int, itemsize: int) -> int:
+def choose_variant(op: str, n_lists: int, n_elements: int, itemsize: int, offsets=None) -> int:
     if chooser is not None:
-        return int(chooser(op, n_lists, n_elements, itemsize))
+        return int(chooser(op, n_lists, n_elements, itemsize, offsets))
     return heuristic_variant(n_lists, n_elements, itemsize)
 
 

@@ -42,10 +42,10 @@ def _fused(array):
     return getattr(array.backend, "fused", False)
 
 
-def _variant(op, array, outlength):
+def _variant(op, array, outlength, offsets=None):
     from awkward_amd import dispatch
 
-    return dispatch.choose_variant(op, outlength, array.length, array.dtype.itemsize)
+    return dispatch.choose_variant(op, outlength, array.length, array.dtype.itemsize, offsets)
 
 
 class Sum(Reducer):
@@ -60,7 +60,7 @@ class Sum(Reducer):
             from awkward_amd import dispatch
 
             array._check(dispatch.reduce_sum(result, array.data, offsets, outlength,
-                                             _variant("sum", array, outlength)))
+                                             _variant("sum", array, outlength, offsets)))
         elif dtype == np.dtype(np.bool_):
             array._check(
                 array.backend["awkward_reduce_sum_int64_bool_64", np.int64, np.bool_, np.int64](
@@ -103,7 +103,7 @@ class _Extremum(Reducer):
 
             outmask = array.nplike.empty(outlength, np.int8) if mask else None
             array._check(dispatch.reduce_extremum(self.name, result, outmask, array.data, offsets, outlength,
-                                                  identity, _variant(self.name, array, outlength)))
+                                                  identity, _variant(self.name, array, outlength, offsets)))
             return type(array)(result, array.backend), outmask
         array._check(
             array.backend[self.kernel, dtype, dtype, np.int64](result, array.data, offsets, outlength, identity)
@@ -140,7 +140,7 @@ class _Arg(Reducer):
 
             outmask = nplike.empty(outlength, np.int8) if mask else None
             array._check(dispatch.reduce_arg(self.name, result, outmask, data, offsets, outlength,
-                                             _variant(self.name, array, outlength)))
+                                             _variant(self.name, array, outlength, offsets)))
             return type(array)(result, array.backend), outmask
         array._check(
             array.backend[self.kernel, np.int64, dtype, np.int64, np.int64](result, data, offsets, starts, outlength)

@@ -241,11 +241,35 @@ def run_config4(ak, torch, steps=3):
     out["autotune"] = runner.tune(reps=3)
     graph = Graph(runner.run)
     out["batched_graph"] = time_form(torch, graph.replay, steps * 3, 1, n_lists)
-    for form in ("plain", "batched_graph"):
+    # the same batch with the tile chosen on the device by the learned dispatcher
+    learned_runner = BatchRunner(dev, variants="learned")
+    lgraph = Graph(learned_runner.run)
+    out["batched_learned_graph"] = time_form(torch, lgraph.replay, steps * 3, 1, n_lists)
+    out["learned_tiles"] = {g.name: g.chosen() for g in learned_runner.groups}
+    for form in ("plain", "batched_graph", "batched_learned_graph"):
         out[form]["lists_per_s"] = out[form].pop("events_per_s")
     out["speedup"] = round(out["batched_graph"]["lists_per_s"] / out["plain"]["lists_per_s"], 2)
+    out["speedup_learned"] = round(out["batched_learned_graph"]["lists_per_s"] / out["plain"]["lists_per_s"], 2)
     ms = out["batched_graph"]["ms_per_step"]
     out["batched_GB/s"] = round(runner.algorithmic_bytes() / ms / 1e6, 1)
+    return out
+
+
+def run_config5(torch, budget_s=40.0):
+    """Learned dispatch: the trained cost model's regret against the
+    exhaustive sweep (every tile timed) on held-out arrays -- seeds 9000+,
+    drawn on the device by the training generators -- beside the autotuner
+    table, the fixed heuristic and the default tile."""
+    from awkward_amd import learned
+
+    device = torch.device("cuda", torch.cuda.current_device())
+    t0 = time.perf_counter()
+    X, T, _ = learned.collect(list(range(9000, 9500)), 20, device, budget_s=budget_s)
+    sweep_s = time.perf_counter() - t0
+    model = learned.Model.load(learned.DEFAULT_MODEL)
+    out = learned.regret(model, X, T, device)
+    out["sweep_s"] = round(sweep_s, 1)
+    out["model"] = {k: model.info.get(k) for k in ("rows", "epochs", "val_regret", "train_s")}
     return out
 
 
@@ -297,6 +321,7 @@ def main():
     ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
     ap.add_argument("--no-config4", action="store_true")
+    ap.add_argument("--no-config5", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each op is independent per "
@@ -368,6 +393,10 @@ def main():
     config4 = None if args.no_config4 else run_config4(ak, torch)
     if config4:
         log(f"config4: plain {config4['plain']['lists_per_s']:.4g} lists/s, batched {config4['batched_graph']['lists_per_s']:.4g} lists/s")
+    config5 = None if args.no_config5 else run_config5(torch)
+    if config5:
+        log(f"config5: learned regret {config5['learned']['aggregate']}, table {config5['table']['aggregate']}, "
+            f"heuristic {config5['heuristic']['aggregate']} over {config5['rows']} held-out rows")
     cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
 
     line = {
@@ -393,6 +422,7 @@ def main():
         "config1": config1,
         "config3": config3,
         "config4": config4,
+        "config5": config5,
     }
     print(json.dumps(line), flush=True)
 

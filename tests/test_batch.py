@@ -60,7 +60,7 @@ def check_array(off, x, res):
     assert same_bits(h[0], res["argsort"].cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(10))
 def test_batch_runner_matches_oracle(variant):
     import torch
 
