@@ -273,6 +273,59 @@ def run_config5(torch, budget_s=40.0):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "event_pipeline_traffic.json")
+KERNEL_SOURCE = os.path.join(ROOT, "awkward_amd", "csrc", "fused.hip")
+
+
+def source_digest():
+    import hashlib
+
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def measured_traffic(events):
+    """HBM bytes per launch of the fused pipeline kernel from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter's reading for a copy of known size), if they were taken on
+    this kernel source and this many events; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("source_sha256") != source_digest() or t.get("events") != events:
+        return None
+    return t.get("traffic_bytes")
+
+
+def traffic_probe(args):
+    """The program the counter runs profile: an elementwise copy of known
+    size (1 GiB read, 1 GiB written), then the fused pipeline launched
+    eagerly, each three times.  Prints the algorithmic bytes."""
+    import torch
+
+    import awkward_amd as ak
+    from awkward_amd.backend import HipBackend
+    from awkward_amd.fused import EventPipeline
+
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    off, fields = make_events(args.events)
+    ev = ak.from_offsets(off, fields, HipBackend(fused=True))
+    pipe = EventPipeline(ev, [("pt", ">", 25.0), ("eta", "abs<", 2.4)], slice(1, None), "pt",
+                         outputs=("num", "sum", "argmax"))
+    for _ in range(3):
+        pipe.run()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_read_bytes": 4 << 28, "copy_write_bytes": 4 << 28, "events": args.events,
+                      "pipeline_alg_bytes": pipe.algorithmic_bytes(int(off[-1])), "source_sha256": source_digest()}),
+          flush=True)
+
+
 def cpu_baseline(ak, off, fields, sample_events):
     import numpy as np
 
@@ -322,11 +375,16 @@ def main():
     ap.add_argument("--no-config3", action="store_true")
     ap.add_argument("--no-config4", action="store_true")
     ap.add_argument("--no-config5", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true", help="only the program profiled by the counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each op is independent per "
               "list; sharding by list ranges is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
 
     import torch
 
@@ -379,7 +437,7 @@ def main():
     alg_bytes = pipe.algorithmic_bytes(N)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.events),
                 "kernel": "event_pipeline_kernel (awkward_hip_event_pipeline)", "kernel_ms": round(kms, 4),
                 "alg_bytes": alg_bytes}
 
