@@ -112,10 +112,8 @@ template <typename T>
 __global__ __launch_bounds__(sortimpl::kBlock) void batch_argsort_kernel(const BatchSortItem* __restrict__ items,
                                                                          int nitems, int64_t total_tiles,
                                                                          unsigned long long* counter, bool asc,
-                                                                         uint64_t* gk0, uint32_t* gi0, uint64_t* gk1,
-                                                                         uint32_t* gi1) {
+                                                                         sortimpl::LongQueue q) {
   using namespace sortimpl;
-  __shared__ SortSmem sm;
   __shared__ int64_t s_tile;
   __shared__ int s_item;
   for (;;) {
@@ -129,10 +127,8 @@ __global__ __launch_bounds__(sortimpl::kBlock) void batch_argsort_kernel(const B
     if (tile >= total_tiles) break;
     const BatchSortItem it = items[s_item];
     __syncthreads();
-    const int64_t e0 = it.elem_begin;
     sort_tile<T, int64_t, true>(it.out, (const T*)it.data, it.length, it.offsets, it.n,
-                                (tile - it.tile_begin) * (int64_t)kBlock, asc, gk0 + e0, gi0 + e0, gk1 + e0,
-                                gi1 + e0, sm);
+                                (tile - it.tile_begin) * (int64_t)kBlock, asc, q, it.elem_begin);
   }
 }
 
@@ -165,6 +161,36 @@ Error launch_batch_variant(int variant, const BatchReduceItem* it, int nitems, i
     case 9: return launch_batch_reduce<T, S, 128, 1, 8192>(it, nitems, tiles, counter, identity, choice, 9);
     default: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, tiles, counter, identity, choice, 0);
   }
+}
+
+// scratch: the long-segment queue and two (key, index) merge buffers over
+// the batch's elements, from the pool.  Segments longer than 4096 elements
+// are sorted by every workgroup after the tile pass (sort.h).
+template <typename T>
+Error batch_argsort(const BatchSortItem* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter,
+                    bool ascending, int64_t total_elements) {
+  using namespace sortimpl;
+  if (nitems <= 0 || total_tiles <= 0) return success();
+  const int64_t ne = total_elements > 0 ? total_elements : 1;
+  const size_t kb = ((size_t)ne * 8 + 255) / 256 * 256;
+  const size_t ib = ((size_t)ne * 4 + 255) / 256 * 256;
+  const int64_t capacity = long_queue_capacity(ne);
+  const size_t qb = ((size_t)capacity * sizeof(LongChunk) + 255) / 256 * 256;
+  char* s = (char*)scratch(256 + qb + 2 * (kb + ib));
+  if (!s) return failure("out of device memory for batch sort scratch", kSliceNone, kSliceNone,
+                         "awkward_amd/csrc/batch.hip");
+  LongQueue q{(LongChunk*)(s + 256), (unsigned long long*)s, (unsigned long long*)(s + 8), capacity};
+  char* b = s + 256 + qb;
+  hipStream_t st = stream();
+  long_queue_reset<><<<1, 1, 0, st>>>(q.count, q.maxlen);
+  int64_t grid = (int64_t)cu_count() * 4;
+  if (grid > total_tiles) grid = total_tiles;
+  batch_argsort_kernel<T><<<(unsigned)grid, kBlock, 0, st>>>(items, nitems, total_tiles, counter, ascending, q);
+  AKHIP_LAUNCH_CHECK("batch_argsort_kernel");
+  const hipError_t e = launch_long_sort<T, true>(q, ne, ascending, (uint64_t*)b, (uint32_t*)(b + 2 * kb),
+                                                 (uint64_t*)(b + kb), (uint32_t*)(b + 2 * kb + ib), st);
+  if (e != hipSuccess) return hip_failure(e, "batch long segment sort");
+  return success();
 }
 
 }  // namespace
@@ -217,23 +243,11 @@ BATCH_REDUCE(float32, float, float)
 BATCH_REDUCE(float64, double, double)
 #undef BATCH_REDUCE
 
-// scratch: 12 bytes x (total elements of the batch) x 2, from the pool.
-#define BATCH_SORT(TN, T)                                                                                        \
-  Error awkward_hip_batch_argsort_##TN(const void* items, int32_t nitems, int64_t total_tiles,                  \
-                                       unsigned long long* counter, bool ascending, int64_t total_elements) {  \
-    if (nitems <= 0 || total_tiles <= 0) return success();                                                    \
-    const size_t kb = ((size_t)(total_elements > 0 ? total_elements : 1) * 8 + 255) / 256 * 256;              \
-    const size_t ib = ((size_t)(total_elements > 0 ? total_elements : 1) * 4 + 255) / 256 * 256;              \
-    char* s = (char*)scratch(2 * (kb + ib));                                                                   \
-    if (!s) return failure("out of device memory for batch sort scratch", kSliceNone, kSliceNone,              \
-                           "awkward_amd/csrc/batch.hip");                                                      \
-    int64_t grid = (int64_t)cu_count() * 4;                                                                    \
-    if (grid > total_tiles) grid = total_tiles;                                                                \
-    batch_argsort_kernel<T><<<(unsigned)grid, sortimpl::kBlock, 0, stream()>>>(                                \
-        (const BatchSortItem*)items, nitems, total_tiles, counter, ascending, (uint64_t*)s, (uint32_t*)(s + 2 * kb), \
-        (uint64_t*)(s + kb), (uint32_t*)(s + 2 * kb + ib));                                                    \
-    AKHIP_LAUNCH_CHECK("batch_argsort_kernel");                                                                \
-    return success();                                                                                          \
+#define BATCH_SORT(TN, T)                                                                                      \
+  Error awkward_hip_batch_argsort_##TN(const void* items, int32_t nitems, int64_t total_tiles,                \
+                                       unsigned long long* counter, bool ascending, int64_t total_elements) { \
+    return batch_argsort<T>((const BatchSortItem*)items, nitems, total_tiles, counter, ascending,            \
+                            total_elements);                                                                 \
   }
 BATCH_SORT(int32, int32_t)
 BATCH_SORT(int64, int64_t)

@@ -9,12 +9,20 @@
 // yields the unique stable order -- the std::stable_sort result; for
 // stable=false the same order is a valid std::sort result.
 //
-// Segments are binned by length inside each workgroup of 256 segments:
+// Segments are binned by length inside each tile of 256 segments:
 //   len <= 16    one thread, a 16-wide bitonic network in registers
 //   len <= 64    one wave, a 64-lane bitonic network over shuffles
-//   len <= 4096  the workgroup, a bitonic network in LDS
-//   longer       the workgroup: 4096-element LDS runs, then merge-path
-//                passes through a global scratch buffer
+//   longer       queued: the segment's 4096-element chunks go to a device
+//                queue (one atomic per wave) and every workgroup of the chip
+//                takes chunks: block_sort (16 per thread in registers, then
+//                merge stages in LDS) sorts each chunk -- a segment of one
+//                chunk is done there -- and merge-path passes
+//                (long_merge_kernel), each workgroup producing one
+//                4096-element block of a merged run, finish the longer ones;
+//                the last pass of a segment writes the output.  Pass w exits
+//                at once when no queued segment is longer than w (the maximum
+//                length is kept on the device), so the host never
+//                synchronises.
 #pragma once
 
 #include <type_traits>
@@ -69,20 +77,8 @@ __device__ inline void store_sorted(void* out, const T* from, int64_t a, int64_t
 }
 
 // ---- len <= 16: registers ------------------------------------------------------
-template <typename T, bool ARG>
-__device__ void sort_thread(void* out, const T* from, int64_t a, int len, bool asc) {
-  uint64_t k[16];
-  uint32_t ix[16];
-#pragma unroll
-  for (int u = 0; u < 16; u++) {
-    if (u < len) {
-      k[u] = order_key<T>(from[a + u], asc);
-      ix[u] = (uint32_t)u;
-    } else {
-      k[u] = kPadKey;
-      ix[u] = kPadIdx;
-    }
-  }
+// 16 (key, index) pairs in registers into ascending order (bitonic network).
+__device__ inline void bitonic16(uint64_t* k, uint32_t* ix) {
 #pragma unroll
   for (int kk = 2; kk <= 16; kk <<= 1) {
 #pragma unroll
@@ -105,6 +101,23 @@ __device__ void sort_thread(void* out, const T* from, int64_t a, int len, bool a
       }
     }
   }
+}
+
+template <typename T, bool ARG>
+__device__ void sort_thread(void* out, const T* from, int64_t a, int len, bool asc) {
+  uint64_t k[16];
+  uint32_t ix[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    if (u < len) {
+      k[u] = order_key<T>(from[a + u], asc);
+      ix[u] = (uint32_t)u;
+    } else {
+      k[u] = kPadKey;
+      ix[u] = kPadIdx;
+    }
+  }
+  bitonic16(k, ix);
 #pragma unroll
   for (int u = 0; u < 16; u++)
     if (u < len) store_sorted<T, ARG>(out, from, a, u, ix[u]);
@@ -136,41 +149,90 @@ __device__ void sort_wave(void* out, const T* from, int64_t a, int len, bool asc
   if (lane < len) store_sorted<T, ARG>(out, from, a, lane, idx);
 }
 
-// ---- len <= kCap: the workgroup, LDS bitonic -----------------------------------------
+// ---- len <= kCap: the workgroup, registers + LDS merges ------------------------------
+// LDS position of element i: one pad slot per 16 elements, so the 16-element
+// rows the lanes of a wave read and write fall in different banks.
+__device__ inline int pad16(int i) { return i + (i >> 4); }
+constexpr int kCapPad = kCap + kCap / 16;
+
+// Sorts n <= kCap pairs (order_key(from[a + i]), idx_base + i) of the
+// workgroup: each thread sorts 16 in registers, then log2(np / 16) merge
+// stages in LDS, each thread finding its 16 outputs' merge-path split by
+// binary search (np = n rounded up to a power of two, at least 16).  The
+// sorted pairs end in s_key / s_idx at positions pad16(0 .. n-1).
 template <typename T>
-__device__ void lds_bitonic(uint64_t* s_key, uint32_t* s_idx, const T* from, int64_t a, int len, bool asc,
-                            uint32_t idx_base) {
-  int np = 1;
-  while (np < len) np <<= 1;
-  for (int i = threadIdx.x; i < np; i += kBlock) {
-    if (i < len) {
-      s_key[i] = order_key<T>(from[a + i], asc);
-      s_idx[i] = idx_base + (uint32_t)i;
+__device__ void block_sort(uint64_t* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
+                           uint32_t idx_base) {
+  constexpr int PER = 16;
+  int np = PER;
+  while (np < n) np <<= 1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < np; i += kBlock) {
+    const int p = pad16(i);
+    if (i < n) {
+      s_key[p] = order_key<T>(from[a + i], asc);
+      s_idx[p] = idx_base + (uint32_t)i;
     } else {
-      s_key[i] = kPadKey;
-      s_idx[i] = kPadIdx;
+      s_key[p] = kPadKey;
+      s_idx[p] = kPadIdx;
     }
   }
   __syncthreads();
-  for (int kk = 2; kk <= np; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < np; i += kBlock) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & kk) == 0;
-          const uint64_t ki = s_key[i], kl = s_key[l];
-          const uint32_t ii = s_idx[i], il = s_idx[l];
-          const bool swap = up ? pair_less(kl, il, ki, ii) : pair_less(ki, ii, kl, il);
-          if (swap) {
-            s_key[i] = kl;
-            s_key[l] = ki;
-            s_idx[i] = il;
-            s_idx[l] = ii;
-          }
-        }
-      }
-      __syncthreads();
+  const bool active = tid * PER < np;
+  uint64_t k[PER];
+  uint32_t x[PER];
+  const int o = tid * PER;  // this thread's first position
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      k[u] = s_key[pad16(o + u)];
+      x[u] = s_idx[pad16(o + u)];
     }
+    bitonic16(k, x);
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      s_key[pad16(o + u)] = k[u];
+      s_idx[pad16(o + u)] = x[u];
+    }
+  }
+  __syncthreads();
+  for (int r = PER; r < np; r <<= 1) {
+    if (active) {
+      const int ps = o / (2 * r) * (2 * r);
+      const int d = o - ps;
+      const int A0 = ps, B0 = ps + r;
+      int lo = d > r ? d - r : 0;
+      int hi = d < r ? d : r;
+      while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        const int j = d - i;
+        const int pa = pad16(A0 + i), pb = pad16(B0 + j - 1);
+        if (pair_less(s_key[pa], s_idx[pa], s_key[pb], s_idx[pb])) lo = i + 1;
+        else hi = i;
+      }
+      int xa = lo, yb = d - lo;
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int pa = pad16(A0 + (xa < r ? xa : r - 1));
+        const int pb = pad16(B0 + (yb < r ? yb : r - 1));
+        const uint64_t ka = s_key[pa], kb = s_key[pb];
+        const uint32_t ia = s_idx[pa], ib = s_idx[pb];
+        const bool from_a = yb >= r || (xa < r && !pair_less(kb, ib, ka, ia));
+        k[u] = from_a ? ka : kb;
+        x[u] = from_a ? ia : ib;
+        xa += from_a ? 1 : 0;
+        yb += from_a ? 0 : 1;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        s_key[pad16(o + u)] = k[u];
+        s_idx[pad16(o + u)] = x[u];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -191,78 +253,65 @@ __device__ inline int64_t merge_corank(const uint64_t* ka, const uint32_t* ia, i
   return lo;
 }
 
-template <typename T, bool ARG>
-__device__ void sort_long(void* out, const T* from, int64_t a, int64_t len, bool asc, uint64_t* s_key,
-                          uint32_t* s_idx, uint64_t* gk0, uint32_t* gi0, uint64_t* gk1, uint32_t* gi1) {
-  // runs of kCap sorted in LDS, written to buffer 0 (segment-relative)
-  for (int64_t c = 0; c < len; c += kCap) {
-    const int n = (int)((len - c) < kCap ? (len - c) : kCap);
-    lds_bitonic<T>(s_key, s_idx, from, a + c, n, asc, (uint32_t)c);
-    for (int i = threadIdx.x; i < n; i += kBlock) {
-      gk0[a + c + i] = s_key[i];
-      gi0[a + c + i] = s_idx[i];
-    }
-    __syncthreads();
-  }
-  uint64_t *sk = gk0, *dk = gk1;
-  uint32_t *si = gi0, *di = gi1;
-  for (int64_t w = kCap; w < len; w <<= 1) {
-    for (int64_t s = 0; s < len; s += 2 * w) {
-      const int64_t na = (len - s) < w ? (len - s) : w;
-      const int64_t nb = (len - s - na) < w ? (len - s - na) : w;
-      const uint64_t* ka = sk + a + s;
-      const uint32_t* ia = si + a + s;
-      const uint64_t* kb = ka + na;
-      const uint32_t* ib = ia + na;
-      const int64_t total = na + nb;
-      const int64_t per = (total + kBlock - 1) / kBlock;
-      const int64_t d0 = (int64_t)threadIdx.x * per < total ? (int64_t)threadIdx.x * per : total;
-      const int64_t d1 = d0 + per < total ? d0 + per : total;
-      int64_t x = merge_corank(ka, ia, na, kb, ib, nb, d0);
-      int64_t y = d0 - x;
-      for (int64_t d = d0; d < d1; d++) {
-        const bool from_a = y >= nb || (x < na && !pair_less(kb[y], ib[y], ka[x], ia[x]));
-        if (from_a) {
-          dk[a + s + d] = ka[x];
-          di[a + s + d] = ia[x];
-          x++;
-        } else {
-          dk[a + s + d] = kb[y];
-          di[a + s + d] = ib[y];
-          y++;
-        }
-      }
-    }
-    __syncthreads();
-    __threadfence_block();
-    uint64_t* tk = sk;
-    sk = dk;
-    dk = tk;
-    uint32_t* ti = si;
-    si = di;
-    di = ti;
-  }
-  for (int64_t r = threadIdx.x; r < len; r += kBlock) store_sorted<T, ARG>(out, from, a, r, si[a + r]);
-  __syncthreads();
-}
-
-struct SortSmem {
-  uint64_t key[kCap];
-  uint32_t idx[kCap];
-  uint64_t mask[kBlock / kWave][2];
+// One 4096-element chunk of a queued long segment.
+struct LongChunk {
+  void* out;         // output array (argsort: int64 local positions; sort: values)
+  const void* from;  // input array
+  int64_t a;         // segment start in the arrays
+  int64_t len;       // segment length
+  int64_t buf;       // segment start in the merge buffers
+  int64_t chunk;     // chunk index within the segment
 };
 
+// The queue: capacity >= sum over queued segments of ceil(len / kCap), at
+// most elements / (kWave + 1) + 2 * elements / kCap since every queued
+// segment exceeds kWave elements.
+inline int64_t long_queue_capacity(int64_t elements) {
+  return elements / (kWave + 1) + 2 * ((elements + kCap - 1) / kCap) + 2;
+}
+
+struct LongQueue {
+  LongChunk* chunks;
+  unsigned long long* count;   // chunks queued
+  unsigned long long* maxlen;  // longest queued segment
+  int64_t capacity;
+};
+
+// Called by every lane of a wave: the lanes with want set append their
+// segment's chunks; one atomic per wave reserves the room.
+__device__ inline void queue_segments(const LongQueue& q, bool want, void* out, const void* from, int64_t a,
+                                      int64_t len, int64_t buf) {
+  if (!__any(want)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long nch = want ? (unsigned long long)((len + kCap - 1) / kCap) : 0ull;
+  const unsigned long long incl = wave_inclusive_scan(nch);
+  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+  unsigned long long longest = want ? (unsigned long long)len : 0ull;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(longest, d, kWave);
+    longest = o > longest ? o : longest;
+  }
+  unsigned long long base = 0;
+  if (lane == 0) {
+    base = atomicAdd(q.count, total);
+    atomicMax(q.maxlen, longest);
+  }
+  base = __shfl(base, 0, kWave);
+  const int64_t first = (int64_t)(base + incl - nch);
+  for (int64_t c = 0; c < (int64_t)nch && first + c < q.capacity; c++)
+    q.chunks[first + c] = LongChunk{out, from, a, len, buf, c};
+}
+
+
 // One tile: segments [S0, S0 + kBlock) of `offsets` (clipped to nseg), each
-// segment sorted into `out` (argsort: local positions; sort: values).
-// gk*/gi* are merge buffers over the element positions (long segments only).
-// Ends with a barrier, so a persistent caller may reuse `sm` at once.
+// segment of at most kWave elements sorted into `out` (argsort: local
+// positions; sort: values); longer ones are queued (merge-buffer position
+// buf_base + segment start).
 template <typename T, typename U, bool ARG>
 __device__ void sort_tile(void* out, const T* __restrict__ from, int64_t length, const U* __restrict__ offsets,
-                          int64_t nseg, int64_t S0, bool asc, uint64_t* gk0, uint32_t* gi0, uint64_t* gk1,
-                          uint32_t* gi1, SortSmem& sm) {
+                          int64_t nseg, int64_t S0, bool asc, const LongQueue& q, int64_t buf_base) {
   const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int wave = tid / kWave;
   const int64_t seg = S0 + tid;
   int64_t a = 0, len = 0;
   if (seg < nseg) {
@@ -272,6 +321,7 @@ __device__ void sort_tile(void* out, const T* __restrict__ from, int64_t length,
     if (b > length) b = length;
     len = b > a ? b - a : 0;
   }
+  queue_segments(q, len > kWave, out, from, a, len, buf_base + a);
   if (len > 0 && len <= 16) sort_thread<T, ARG>(out, from, a, (int)len, asc);
   uint64_t mid = __ballot(len > 16 && len <= kWave);
   while (mid) {
@@ -281,35 +331,210 @@ __device__ void sort_tile(void* out, const T* __restrict__ from, int64_t length,
     const int sl = (int)__shfl(len, b, kWave);
     sort_wave<T, ARG>(out, from, sa, sl, asc);
   }
-  const uint64_t big = __ballot(len > kWave && len <= kCap);
-  const uint64_t huge = __ballot(len > kCap);
-  if (lane == 0) {
-    sm.mask[wave][0] = big;
-    sm.mask[wave][1] = huge;
+}
+
+// ---- queued long segments -------------------------------------------------------------
+// (a template so every translation unit may instantiate it)
+template <int Dummy = 0>
+__global__ void long_queue_reset(unsigned long long* count, unsigned long long* maxlen) {
+  *count = 0ull;
+  *maxlen = 0ull;
+}
+
+// Every chunk sorted in LDS: a segment of one chunk straight to its output,
+// longer ones into merge buffer 0 (segment-relative positions).
+template <typename T, bool ARG>
+__global__ __launch_bounds__(kBlock) void long_runs_kernel(const LongChunk* __restrict__ chunks,
+                                                           const unsigned long long* __restrict__ count,
+                                                           int64_t capacity, bool asc, uint64_t* __restrict__ gk,
+                                                           uint32_t* __restrict__ gi) {
+  __shared__ uint64_t s_key[kCapPad];
+  __shared__ uint32_t s_idx[kCapPad];
+  int64_t nc = (int64_t)*count;
+  if (nc > capacity) nc = capacity;
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const LongChunk ch = chunks[c];
+    const int64_t c0 = ch.chunk * kCap;
+    const int n = (int)((ch.len - c0) < kCap ? (ch.len - c0) : kCap);
+    block_sort<T>(s_key, s_idx, (const T*)ch.from, ch.a + c0, n, asc, (uint32_t)c0);
+    if (ch.len <= kCap) {
+      for (int i = threadIdx.x; i < n; i += kBlock)
+        store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, s_idx[pad16(i)]);
+    } else {
+      for (int i = threadIdx.x; i < n; i += kBlock) {
+        gk[ch.buf + c0 + i] = s_key[pad16(i)];
+        gi[ch.buf + c0 + i] = s_idx[pad16(i)];
+      }
+    }
+    __syncthreads();
   }
+}
+
+// Co-ranks of two diagonals at once: threads [0, 128) search diag[0],
+// threads [128, 256) diag[1]; each round tests 128 positions of the
+// remaining range, so a range of n takes about log_128(n) rounds of
+// parallel loads.  P(i) = "A[i] sorts before B[diag - i - 1]" is true below
+// the co-rank and false from it on.
+__device__ inline void corank_pair(const uint64_t* ka, const uint32_t* ia, int64_t na, const uint64_t* kb,
+                                   const uint32_t* ib, int64_t nb, const int64_t* diag, int64_t* result,
+                                   int* s_cnt) {
+  constexpr int G = kBlock / 2;
+  const int g = threadIdx.x / G;
+  const int t = threadIdx.x % G;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int64_t d = diag[g];
+  int64_t lo = d > nb ? d - nb : 0;
+  int64_t hi = d < na ? d : na;
+  for (;;) {
+    const bool done = lo >= hi;
+    if (__syncthreads_and(done ? 1 : 0)) break;
+    const int64_t step = (hi - lo + G) / G;
+    const int64_t i = lo + (int64_t)t * step;
+    bool pred = false;
+    if (!done && i < hi) {
+      const int64_t j = d - i;
+      pred = j > 0 && i < na && pair_less(ka[i], ia[i], kb[j - 1], ib[j - 1]);
+    }
+    const uint64_t bal = __ballot(pred);
+    if (lane == 0) s_cnt[wave] = __popcll(bal);
+    __syncthreads();
+    const int k = s_cnt[2 * g] + s_cnt[2 * g + 1];
+    if (!done) {
+      const int64_t nlo = k > 0 ? lo + (int64_t)(k - 1) * step + 1 : lo;
+      const int64_t cand = lo + (int64_t)k * step;
+      hi = cand < hi ? cand : hi;
+      lo = nlo;
+    }
+    __syncthreads();  // s_cnt is rewritten by the next round
+  }
+  if (t == 0) result[g] = lo;
   __syncthreads();
-  for (int w = 0; w < kBlock / kWave; w++) {
-    for (int kind = 0; kind < 2; kind++) {
-      uint64_t m = sm.mask[w][kind];
-      while (m) {
-        const int b = __builtin_ctzll(m);
-        m &= m - 1;
-        const int64_t sgi = S0 + w * kWave + b;
-        int64_t sa = (int64_t)offsets[sgi], sb = (int64_t)offsets[sgi + 1];
-        if (sa > length) sa = length;
-        if (sb > length) sb = length;
-        const int64_t sl = sb - sa;
-        if (kind == 0) {
-          lds_bitonic<T>(sm.key, sm.idx, from, sa, (int)sl, asc, 0);
-          for (int i = tid; i < sl; i += kBlock) store_sorted<T, ARG>(out, from, sa, i, sm.idx[i]);
-          __syncthreads();
-        } else {
-          sort_long<T, ARG>(out, from, sa, sl, asc, sm.key, sm.idx, gk0, gi0, gk1, gi1);
+}
+
+// Merge pass over runs of width w: the workgroup for chunk c produces
+// elements [c * kCap, (c + 1) * kCap) of the merged pair of runs holding
+// them; the pass that completes a segment writes its output.
+template <typename T, bool ARG>
+__global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __restrict__ chunks,
+                                                            const unsigned long long* __restrict__ count,
+                                                            const unsigned long long* __restrict__ maxlen,
+                                                            int64_t capacity, int64_t w,
+                                                            const uint64_t* __restrict__ sk,
+                                                            const uint32_t* __restrict__ si, uint64_t* __restrict__ dk,
+                                                            uint32_t* __restrict__ di) {
+  if ((int64_t)*maxlen <= w) return;  // every queued segment is already one run
+  constexpr int PER = kCap / kBlock;   // outputs per thread
+  __shared__ uint64_t s_key[kCap];
+  __shared__ uint32_t s_idx[kCap];
+  __shared__ int64_t s_diag[2], s_co[2];
+  __shared__ int s_cnt[kBlock / kWave];
+  int64_t nc = (int64_t)*count;
+  if (nc > capacity) nc = capacity;
+  const int tid = threadIdx.x;
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const LongChunk ch = chunks[c];
+    if (ch.len <= w) continue;  // workgroup-uniform
+    const int64_t d0 = ch.chunk * kCap;
+    const int64_t d1 = (d0 + kCap) < ch.len ? d0 + kCap : ch.len;
+    const int64_t ps = d0 / (2 * w) * (2 * w);
+    const int64_t na = (ch.len - ps) < w ? ch.len - ps : w;
+    const int64_t rest = ch.len - ps - na;
+    const int64_t nb = rest < w ? rest : w;
+    const uint64_t* ka = sk + ch.buf + ps;
+    const uint32_t* ia = si + ch.buf + ps;
+    const uint64_t* kb = ka + na;
+    const uint32_t* ib = ia + na;
+    if (tid == 0) {
+      s_diag[0] = d0 - ps;
+      s_diag[1] = d1 - ps;
+    }
+    __syncthreads();
+    corank_pair(ka, ia, na, kb, ib, nb, s_diag, s_co, s_cnt);
+    const int64_t x0 = s_co[0], x1 = s_co[1];
+    const int64_t y0 = s_diag[0] - x0, y1 = s_diag[1] - x1;
+    const int nal = (int)(x1 - x0), nbl = (int)(y1 - y0);
+    const int n = nal + nbl;
+    for (int i = tid; i < n; i += kBlock) {
+      if (i < nal) {
+        s_key[i] = ka[x0 + i];
+        s_idx[i] = ia[x0 + i];
+      } else {
+        s_key[i] = kb[y0 + i - nal];
+        s_idx[i] = ib[y0 + i - nal];
+      }
+    }
+    __syncthreads();
+    // this thread's outputs [tid * PER, tid * PER + PER) of the block
+    uint64_t rk[PER];
+    uint32_t ri[PER];
+    const int o0 = tid * PER < n ? tid * PER : n;
+    const int o1 = o0 + PER < n ? o0 + PER : n;
+    {
+      int x = (int)merge_corank(s_key, s_idx, nal, s_key + nal, s_idx + nal, nbl, o0);
+      int y = o0 - x;
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        if (o0 + u < o1) {
+          const bool from_a =
+              y >= nbl || (x < nal && !pair_less(s_key[nal + y], s_idx[nal + y], s_key[x], s_idx[x]));
+          if (from_a) {
+            rk[u] = s_key[x];
+            ri[u] = s_idx[x];
+            x++;
+          } else {
+            rk[u] = s_key[nal + y];
+            ri[u] = s_idx[nal + y];
+            y++;
+          }
         }
       }
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      if (o0 + u < o1) {
+        s_key[o0 + u] = rk[u];
+        s_idx[o0 + u] = ri[u];
+      }
+    }
+    __syncthreads();
+    if (2 * w >= ch.len) {
+      for (int i = tid; i < n; i += kBlock) store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, d0 + i, s_idx[i]);
+    } else {
+      for (int i = tid; i < n; i += kBlock) {
+        dk[ch.buf + d0 + i] = s_key[i];
+        di[ch.buf + d0 + i] = s_idx[i];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
+}
+
+// Sorts everything queued: runs, then merge passes up to runs of max_len
+// (an upper bound of the longest segment known on the host).  Buffers
+// (gk0, gi0) and (gk1, gi1) cover the merge-buffer positions.
+template <typename T, bool ARG>
+inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc, uint64_t* gk0, uint32_t* gi0,
+                                   uint64_t* gk1, uint32_t* gi1, hipStream_t st) {
+  if (max_len <= kWave) return hipSuccess;
+  int64_t grid = (int64_t)cu_count() * 8;
+  if (grid > q.capacity) grid = q.capacity;
+  if (grid < 1) grid = 1;
+  long_runs_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.capacity, asc, gk0, gi0);
+  uint64_t *sk = gk0, *dk = gk1;
+  uint32_t *si = gi0, *di = gi1;
+  for (int64_t w = kCap; w < max_len; w <<= 1) {
+    long_merge_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
+                                                                 dk, di);
+    uint64_t* tk = sk;
+    sk = dk;
+    dk = tk;
+    uint32_t* ti = si;
+    si = di;
+    di = ti;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace sortimpl

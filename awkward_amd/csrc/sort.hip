@@ -16,10 +16,8 @@ const char* kFile = "awkward_amd/csrc/sort.hip";
 template <typename T, typename U, bool ARG>
 __global__ __launch_bounds__(kBlock) void segmented_sort_kernel(void* out, const T* __restrict__ from, int64_t length,
                                                                 const U* __restrict__ offsets, int64_t nseg,
-                                                                bool asc, uint64_t* gk0, uint32_t* gi0,
-                                                                uint64_t* gk1, uint32_t* gi1) {
-  __shared__ SortSmem sm;
-  sort_tile<T, U, ARG>(out, from, length, offsets, nseg, (int64_t)blockIdx.x * kBlock, asc, gk0, gi0, gk1, gi1, sm);
+                                                                bool asc, LongQueue q) {
+  sort_tile<T, U, ARG>(out, from, length, offsets, nseg, (int64_t)blockIdx.x * kBlock, asc, q, 0);
 }
 
 // Positions outside every segment: argsort leaves the identity index there,
@@ -44,21 +42,39 @@ __global__ void uncovered_kernel(void* out, const T* __restrict__ from, int64_t 
 }
 
 // Scratch layout: 256 bytes of head room (local_preparenext keeps its
-// single-segment offsets there), then two (key, index) merge buffers over
-// the element positions.
-inline char* sort_scratch(int64_t length, size_t* kbytes, size_t* ibytes) {
-  *kbytes = ((size_t)length * 8 + 255) / 256 * 256;
-  *ibytes = ((size_t)length * 4 + 255) / 256 * 256;
-  return (char*)scratch(256 + 2 * (*kbytes + *ibytes));
+// single-segment offsets there), 256 bytes of queue counters, the long-chunk
+// queue, then two (key, index) merge buffers over the element positions.
+struct SortScratch {
+  char* head;
+  LongQueue q;
+  uint64_t *gk0, *gk1;
+  uint32_t *gi0, *gi1;
+};
+
+inline bool sort_scratch(int64_t length, SortScratch* out) {
+  const size_t kbytes = ((size_t)length * 8 + 255) / 256 * 256;
+  const size_t ibytes = ((size_t)length * 4 + 255) / 256 * 256;
+  const int64_t capacity = long_queue_capacity(length);
+  const size_t qbytes = ((size_t)capacity * sizeof(LongChunk) + 255) / 256 * 256;
+  char* s = (char*)scratch(512 + qbytes + 2 * (kbytes + ibytes));
+  if (!s) return false;
+  out->head = s;
+  out->q.count = (unsigned long long*)(s + 256);
+  out->q.maxlen = (unsigned long long*)(s + 256 + 8);
+  out->q.chunks = (LongChunk*)(s + 512);
+  out->q.capacity = capacity;
+  char* b = s + 512 + qbytes;
+  out->gk0 = (uint64_t*)b;
+  out->gk1 = (uint64_t*)(b + kbytes);
+  out->gi0 = (uint32_t*)(b + 2 * kbytes);
+  out->gi1 = (uint32_t*)(b + 2 * kbytes + ibytes);
+  return true;
 }
 
 template <typename T, typename U, bool ARG>
-Error segmented_sort(void* out, const T* from, int64_t length, const U* offsets, int64_t offsetslength, bool asc) {
+Error segmented_sort(void* out, const T* from, int64_t length, const U* offsets, int64_t offsetslength, bool asc,
+                     const SortScratch& sc) {
   if (length <= 0) return success();
-  size_t kbytes, ibytes;
-  char* s = sort_scratch(length, &kbytes, &ibytes);
-  if (!s) return failure("out of device memory for sort scratch", kSliceNone, kSliceNone, kFile);
-  s += 256;
   hipStream_t st = stream();
   uncovered_kernel<T, U, ARG><<<grid_for(length, 256, 16), 256, 0, st>>>(out, from, length, offsets, offsetslength);
   const int64_t nseg = offsetslength - 1;
@@ -66,14 +82,21 @@ Error segmented_sort(void* out, const T* from, int64_t length, const U* offsets,
     AKHIP_LAUNCH_CHECK("uncovered_kernel");
     return success();
   }
-  uint64_t* gk0 = (uint64_t*)s;
-  uint64_t* gk1 = (uint64_t*)(s + kbytes);
-  uint32_t* gi0 = (uint32_t*)(s + 2 * kbytes);
-  uint32_t* gi1 = (uint32_t*)(s + 2 * kbytes + ibytes);
+  long_queue_reset<><<<1, 1, 0, st>>>(sc.q.count, sc.q.maxlen);
   segmented_sort_kernel<T, U, ARG><<<(unsigned)ceil_div(nseg, kBlock), kBlock, 0, st>>>(out, from, length, offsets,
-                                                                                       nseg, asc, gk0, gi0, gk1, gi1);
+                                                                                       nseg, asc, sc.q);
   AKHIP_LAUNCH_CHECK("segmented_sort_kernel");
+  const hipError_t e = launch_long_sort<T, ARG>(sc.q, length, asc, sc.gk0, sc.gi0, sc.gk1, sc.gi1, st);
+  if (e != hipSuccess) return hip_failure(e, "long segment sort");
   return success();
+}
+
+template <typename T, typename U, bool ARG>
+Error segmented_sort(void* out, const T* from, int64_t length, const U* offsets, int64_t offsetslength, bool asc) {
+  if (length <= 0) return success();
+  SortScratch sc;
+  if (!sort_scratch(length, &sc)) return failure("out of device memory for sort scratch", kSliceNone, kSliceNone, kFile);
+  return segmented_sort<T, U, ARG>(out, from, length, offsets, offsetslength, asc, sc);
 }
 
 // awkward_ListOffsetArray_local_preparenext_64: the stable argsort of one
@@ -126,11 +149,11 @@ SORT(float64, double)
 
 Error awkward_ListOffsetArray_local_preparenext_64(int64_t* tocarry, const int64_t* fromindex, int64_t length) {
   if (length <= 0) return success();
-  size_t kbytes, ibytes;
-  int64_t* off = (int64_t*)sort_scratch(length, &kbytes, &ibytes);  // head room holds the offsets pair
-  if (!off) return failure("out of device memory", kSliceNone, kSliceNone, kFile);
+  SortScratch sc;
+  if (!sort_scratch(length, &sc)) return failure("out of device memory", kSliceNone, kSliceNone, kFile);
+  int64_t* off = (int64_t*)sc.head;  // head room holds the offsets pair
   single_segment_offsets<<<1, 1, 0, stream()>>>(off, length);
-  return segmented_sort<int64_t, int64_t, true>(tocarry, fromindex, length, off, 2, true);
+  return segmented_sort<int64_t, int64_t, true>(tocarry, fromindex, length, off, 2, true, sc);
 }
 
 }  // extern "C"

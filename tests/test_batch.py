@@ -102,3 +102,27 @@ def test_batch_graph_replays():
                 assert torch.equal(v, got[k]), (i, k)
     timings = runner.tune(reps=1)
     assert set(timings) <= {"int32", "int64", "float32", "float64"}
+
+
+def test_batch_long_lists():
+    """Lists of 5e3..1e5 elements: the queued long-segment sort across
+    arrays, and the reductions' long-list path."""
+    import torch
+
+    from awkward_amd.batch import BatchRunner
+
+    rng = np.random.default_rng(60)
+    batch = []
+    for k, dt in enumerate([np.float32, np.int64, np.float64, np.int32, np.float32]):
+        counts = rng.integers(4000, 100001, 6 + k)
+        counts[0] = 4097
+        off = offsets_from_counts(counts)
+        N = int(off[-1])
+        x = rng.integers(-1000, 1000, N).astype(dt) if np.dtype(dt).kind == "i" else rng.standard_normal(N).astype(dt)
+        batch.append((off, x))
+    dev = [(torch.from_numpy(o).cuda(), torch.from_numpy(x).cuda()) for o, x in batch]
+    runner = BatchRunner(dev)
+    runner.run()
+    torch.cuda.synchronize()
+    for i, (o, x) in enumerate(batch):
+        check_array(o, x, runner.result(i))

@@ -312,8 +312,26 @@ def test_argsort_uncovered_and_clamped():
 
 def test_local_preparenext():
     rng = np.random.default_rng(23)
-    x = rng.integers(0, 50, 10000).astype(np.int64)
-    h, d, _, _ = run_both("awkward_ListOffsetArray_local_preparenext_64", [np.zeros(10000, np.int64), x, 10000])
+    for n in (10000, 300000):  # the second is one queued long segment
+        x = rng.integers(0, 50, n).astype(np.int64)
+        h, d, _, _ = run_both("awkward_ListOffsetArray_local_preparenext_64", [np.zeros(n, np.int64), x, n])
+        assert same_bits(h[0], d[0]), n
+
+
+@pytest.mark.parametrize("t", ["int32", "int64", "float32", "float64"])
+@pytest.mark.parametrize("ascending", [True, False])
+def test_long_segments(t, ascending):
+    """Segments past 4096 elements go through the queued multi-workgroup
+    merge sort: exact multiples of the chunk, one past, and up to 1e5."""
+    rng = np.random.default_rng(24)
+    counts = np.array([8192, 12288, 4097, 100000, 3, 0, 65537, 5000, 33000, 16384, 8193])
+    rng.shuffle(counts)
+    off = offsets_from_counts(counts)
+    x = values(rng, t, int(off[-1]))
+    h, d, _, _ = run_both(f"awkward_argsort_{t}_int64", [np.zeros(len(x), np.int64), x, len(x), off, len(off),
+                                                         ascending, True])
+    assert same_bits(h[0], d[0])
+    h, d, _, _ = run_both(f"awkward_sort_{t}", [np.zeros(len(x), x.dtype), x, len(x), off, len(off), ascending, True])
     assert same_bits(h[0], d[0])
 
 
