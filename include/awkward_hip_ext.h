@@ -1,0 +1,197 @@
+/* awkward_hip_ext.h -- entry points of libawkward_hip.so that have no
+ * counterpart in awkward-cpp.  They serve the same results as sequences of
+ * the registry kernels declared in awkward_hip.h (tuned tile variants with
+ * fused epilogues, fused op sequences, the batch runner, the learned tile
+ * dispatcher) or are runtime plumbing.  Same conventions: device pointers
+ * unless a comment says host, kernels enqueued on the stream set by
+ * awkward_hip_set_stream(), errors returned as struct Error (str == NULL is
+ * success).  Every symbol declared here is exported by the library.
+ */
+#ifndef AWKWARD_HIP_EXT_H
+#define AWKWARD_HIP_EXT_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#include "awkward_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ------------------------------------------------------------ */
+/* Grow the per-device scratch pool to at least `bytes` now (before capturing
+   a hipGraph whose kernels use scratch).  0 on success. */
+int awkward_hip_reserve_scratch(int64_t bytes);
+/* Number of reduce tile variants (threads x lists per thread x LDS stage). */
+int awkward_hip_reduce_variant_count(void);
+
+/* ---- tuned reductions: one tile variant (0 .. 9), fused epilogues ---------
+   sum:  awkward_reduce_sum_<OUT>_<IN>_64 with the variant's tile.
+   max/min_masked: awkward_reduce_max/min + the empty-list mask
+     (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64) in one pass; mask may
+     be NULL.
+   argmax/argmin_local: awkward_reduce_argmax/argmin + adjust_starts with
+     starts = offsets[:-1] (local indices) + the empty-list mask. */
+struct Error awkward_hip_reduce_sum_int64_int8(int64_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint64_uint8(uint64_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int64_int16(int64_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint64_uint16(uint64_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int64_int32(int64_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint64_uint32(uint64_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int64_int64(int64_t* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint64_uint64(uint64_t* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float32_float32(float* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_float64(double* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int32_int8(int32_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint32_uint8(uint32_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int32_int16(int32_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint32_uint16(uint32_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int32_int32(int32_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_uint32_uint32(uint32_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_int8(double* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_uint8(double* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_int16(double* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_uint16(double* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_int32(double* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_uint32(double* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_int64(double* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_uint64(double* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_float64_float32(double* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int64_bool(int64_t* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_sum_int32_bool(int32_t* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_max_masked_int8(int8_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int8_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_uint8(uint8_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, uint8_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_int16(int16_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int16_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_uint16(uint16_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, uint16_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_int32(int32_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int32_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_uint32(uint32_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, uint32_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int64_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_uint64(uint64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, uint64_t identity, int variant);
+struct Error awkward_hip_reduce_max_masked_float32(float* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, float identity, int variant);
+struct Error awkward_hip_reduce_max_masked_float64(double* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, double identity, int variant);
+struct Error awkward_hip_reduce_min_masked_int8(int8_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int8_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_uint8(uint8_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, uint8_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_int16(int16_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int16_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_uint16(uint16_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, uint16_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_int32(int32_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int32_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_uint32(uint32_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, uint32_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int64_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_uint64(uint64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, uint64_t identity, int variant);
+struct Error awkward_hip_reduce_min_masked_float32(float* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, float identity, int variant);
+struct Error awkward_hip_reduce_min_masked_float64(double* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, double identity, int variant);
+struct Error awkward_hip_reduce_argmax_local_int8(int64_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_uint8(int64_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_int16(int64_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_uint16(int64_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_int32(int64_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_uint32(int64_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_uint64(int64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_local_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_int8(int64_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_uint8(int64_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_int16(int64_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_uint16(int64_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_int32(int64_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_uint32(int64_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_uint64(int64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_local_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+
+/* ---- gathers and small host-layer steps ------------------------------------
+   take_<bytes>: out[i] = data[index[i]] (NumpyArray._carry, numpyarray.py:367-373).
+   bool_to_index: the jagged boolean mask -> jagged int index of
+     _slicing._normalise_item_bool_to_int (_slicing.py:476-530): toindex gets
+     the local positions of the True entries (room for len(mask)), tooffsets
+     (n + 1) their per-list counts' prefix.
+   subtract_i64: out = a - b (ak.num's stops - starts, ak_num.py:133-142). */
+struct Error awkward_hip_take_1(void* out, const void* data, const int64_t* index, int64_t n);
+struct Error awkward_hip_take_2(void* out, const void* data, const int64_t* index, int64_t n);
+struct Error awkward_hip_take_4(void* out, const void* data, const int64_t* index, int64_t n);
+struct Error awkward_hip_take_8(void* out, const void* data, const int64_t* index, int64_t n);
+struct Error awkward_hip_bool_to_index(int64_t* toindex, int64_t* tooffsets, const bool* mask, const int64_t* offsets, int64_t n);
+struct Error awkward_hip_subtract_i64(int64_t* out, const int64_t* a, const int64_t* b, int64_t n);
+
+/* ---- combinations without a host round trip ---------------------------------
+   awkward_ListArray_combinations for lists [starts, stops) whose output
+   offsets tooffsets (length + 1, from awkward_ListArray_combinations_length)
+   are already on the device; tocarry: device array of n device pointers;
+   totallen: the number of tuples (tooffsets[length]). */
+struct Error awkward_hip_combinations_from_offsets(int64_t** tocarry, int64_t* toindex, const int64_t* tooffsets, const int64_t* starts, const int64_t* stops, int64_t length, int64_t n, bool replacement, int64_t totallen);
+
+/* ---- fused op sequences (capturable in a hipGraph) ----------------------------
+   event_pipeline: per event of records, sel = ev[AND of terms][:, start:stop:step]
+     then num(sel), sum/argmax/max of the float32 field `red` -- the results of
+     the mask ufuncs + bool_to_index + jagged_expand/carrylen/apply +
+     next_range + gathers + reduce kernels.  field_ptrs, field_types,
+     term_* are HOST arrays; types 0 f32, 1 f64, 2 i8, 3 i32, 4 i64; ops
+     0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 !=; any output may be NULL.
+   pair_max: per event, max and first argmax over its particle pairs (the
+     order of awkward_ListArray_combinations, n = 2) of
+     2 pt_i pt_j (cosh(eta_i - eta_j) - cos(phi_i - phi_j)) in float32.
+   pair_expr: that expression over already gathered legs. */
+struct Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_particles, const void* const* field_ptrs, const int32_t* field_types, int32_t nfields, const int32_t* term_field, const int32_t* term_op, const int32_t* term_abs, const double* term_value, int32_t nterms, int64_t start, int64_t stop, int64_t step, int32_t red, int64_t* out_num, float* out_sum, int64_t* out_argmax, int8_t* out_argmask, float* out_max, int8_t* out_maxmask);
+struct Error awkward_hip_pair_max(const int64_t* offsets, int64_t n, const float* pt, const float* eta, const float* phi, float* out_max, int64_t* out_argmax, int8_t* out_mask, float identity);
+struct Error awkward_hip_pair_expr(float* out, const float* pt1, const float* eta1, const float* phi1, const float* pt2, const float* eta2, const float* phi2, int64_t n);
+
+/* ---- batch runner: many arrays of one dtype, persistent work queues ------------
+   items: device table of 64-byte rows {data, offsets, out_sum, out_min,
+   out_argmin, out_mask, n, tile_begin} (reduce) or 56-byte rows {data,
+   offsets, out, n, length, tile_begin, elem_begin} (argsort), tile_begin
+   ascending in units of the variant's lists per tile; counter: a device
+   uint64 cleared by awkward_hip_batch_reset before each launch.
+   reduce: per list sum (awkward_reduce_sum's output type), min (with the
+   type's identity) and local argmin + empty mask in one pass.
+   reduce_<T>_chosen: all ten tiles enqueued, each predicated on the device
+   int choice[0] (awkward_hip_dispatch_infer); tables / total_tiles are HOST
+   arrays of ten.
+   argsort: stable ascending/descending argsort per list, local indices. */
+struct Error awkward_hip_batch_reset(unsigned long long* counter);
+int64_t awkward_hip_batch_reduce_tile_lists(int variant);
+int64_t awkward_hip_batch_sort_tile_lists(void);
+struct Error awkward_hip_batch_reduce_int32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int32_t identity, int32_t variant);
+struct Error awkward_hip_batch_reduce_int32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int32_t identity, const int32_t* choice);
+struct Error awkward_hip_batch_argsort_int32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_int64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int64_t identity, int32_t variant);
+struct Error awkward_hip_batch_reduce_int64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int64_t identity, const int32_t* choice);
+struct Error awkward_hip_batch_argsort_int64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_float32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, float identity, int32_t variant);
+struct Error awkward_hip_batch_reduce_float32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, float identity, const int32_t* choice);
+struct Error awkward_hip_batch_argsort_float32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_float64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, double identity, int32_t variant);
+struct Error awkward_hip_batch_reduce_float64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, double identity, const int32_t* choice);
+struct Error awkward_hip_batch_argsort_float64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
+
+/* ---- learned tile dispatcher ---------------------------------------------------
+   layout(what): 0 features (16), 1 hidden (64), 2 outputs (11), 3 trained
+     parameters, 4 parameter array length (parameters + normalisation), 5
+     accumulator words per slot, 6 lists per statistics tile, 7 rows per
+     training workgroup.
+   reset / stats: integer statistics of list lengths per slot; items: device
+     rows {offsets, n, tile_begin, slot} (32 bytes).
+   infer: meta (device int32[4 * nslots]) {itemsize, op, fallback, 0} with op
+     0 sum, 1 min/max, 2 argmin/argmax, 3 batched sum+min+argmin; writes
+     features (16 per slot), costs (11 per slot) -- either may be NULL -- and
+     choice (2 per slot: tile, 1 if the fallback was taken).
+   mlp_forward: rows of raw features -> 11 outputs (+ inside-range flags).
+   mlp_train_step / mlp_adam: one minibatch's gradients (one row per
+     workgroup of 64) and the Adam step over their fixed-order sum.
+   dispatch_time_variants: the exhaustive sweep; ms_out is a HOST float[10];
+     dtype 0 int32, 1 int64, 2 float32, 3 float64; synchronises. */
+int32_t awkward_hip_dispatch_layout(int32_t what);
+struct Error awkward_hip_dispatch_reset(unsigned long long* acc, int64_t nslots);
+struct Error awkward_hip_dispatch_stats(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* acc);
+struct Error awkward_hip_dispatch_infer(const unsigned long long* acc, const int32_t* meta, int32_t nslots, const float* params, float* features, float* costs, int32_t* choice);
+struct Error awkward_hip_mlp_forward(const float* X, int64_t rows, const float* params, float* Y, int32_t* inside);
+struct Error awkward_hip_mlp_train_step(const float* X, const float* T, const float* ow, int64_t R, int64_t pa, int64_t pb, int64_t row0, int64_t bsz, const float* params, float* grads, float* losses);
+struct Error awkward_hip_mlp_adam(float* params, float* m, float* v, const float* grads, const float* losses, int32_t nparts, float lr, float beta1, float beta2, float eps, int64_t step, float* loss_acc);
+struct Error awkward_hip_dispatch_time_variants(int32_t op, int32_t dtype, const void* data, const int64_t* offsets, int64_t n, int32_t reps, float* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AWKWARD_HIP_EXT_H */

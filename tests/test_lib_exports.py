@@ -12,8 +12,8 @@ from awkward_amd._lib import LIBPATH
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    with open(os.path.join(ROOT, "include", "awkward_hip.h")) as f:
+def declared(header="awkward_hip.h"):
+    with open(os.path.join(ROOT, "include", header)) as f:
         text = f.read()
     return re.findall(r"^\S.*?\b(awkward_\w+)\(", text, flags=re.M)
 
@@ -26,10 +26,24 @@ def test_header_declares_every_specialization():
 
 
 @pytest.mark.skipif(not os.path.exists(LIBPATH), reason="libawkward_hip.so not built")
-def test_library_exports_every_declared_symbol():
+@pytest.mark.parametrize("header", ["awkward_hip.h", "awkward_hip_ext.h"])
+def test_library_exports_every_declared_symbol(header):
     lib = ctypes.CDLL(LIBPATH)
-    missing = [n for n in declared() if not hasattr(lib, n)]
+    names = declared(header)
+    assert names
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIBPATH), reason="libawkward_hip.so not built")
+def test_every_export_is_declared():
+    """Nothing the library exports under awkward_ is missing from the headers."""
+    import subprocess
+
+    out = subprocess.run(["nm", "-D", "--defined-only", LIBPATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.split() and line.split()[-1].startswith("awkward_")}
+    headers = set(declared("awkward_hip.h")) | set(declared("awkward_hip_ext.h"))
+    assert exported - headers == set()
 
 
 def test_product_does_not_import_oracle():
