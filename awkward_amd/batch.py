@@ -110,14 +110,14 @@ class _Group:
         self.sort_table = torch.from_numpy(np.array(sort_rows, dtype=np.uint64).view(np.int64)).to(device)
         P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         self._reset = _fn("awkward_hip_batch_reset", [P])
-        self._reduce = _fn(f"awkward_hip_batch_reduce_{self.name}", [P, I32, I64, P, _CT[self.name], I32])
+        self._reduce = _fn(f"awkward_hip_batch_reduce_{self.name}", [P, I32, I64, P, _CT[self.name], I32, I64])
         self._sort = _fn(f"awkward_hip_batch_argsort_{self.name}", [P, I32, I64, P, ctypes.c_bool, I64])
         self.identity = min_identity(self.dtype)
         if self.dispatcher is None:
             self.red_table, self.reduce_tiles = self._reduce_table(arrays, int(tile_lists(self.variant)), device)
             return
         # learned: a table per tile shape, the statistics plan of the whole group
-        self._chosen = _fn(f"awkward_hip_batch_reduce_{self.name}_chosen", [P, P, I32, P, _CT[self.name], P])
+        self._chosen = _fn(f"awkward_hip_batch_reduce_{self.name}_chosen", [P, P, I32, P, _CT[self.name], P, I64])
         self.var_tables = [self._reduce_table(arrays, int(tile_lists(v)), device) for v in range(len(REDUCE_VARIANTS))]
         self._ptrs = (ctypes.c_void_p * len(REDUCE_VARIANTS))(*[t.data_ptr() for t, _ in self.var_tables])
         self._tiles = (ctypes.c_int64 * len(REDUCE_VARIANTS))(*[n for _, n in self.var_tables])
@@ -134,11 +134,11 @@ class _Group:
         _check(self._reset(self.counter.data_ptr()))
         if self.dispatcher is None:
             _check(self._reduce(self.red_table.data_ptr(), n, self.reduce_tiles, self.counter.data_ptr(),
-                                self.identity, self.variant))
+                                self.identity, self.variant, self.elements))
             return
         self.dispatcher.run_device(self.plan, self.meta, self.choice)
         _check(self._chosen(ctypes.addressof(self._ptrs), ctypes.addressof(self._tiles), n, self.counter.data_ptr(),
-                            self.identity, self.choice.data_ptr()))
+                            self.identity, self.choice.data_ptr(), self.elements))
 
     def chosen(self):
         """The learned tile (reads the device choice: synchronises)."""
@@ -173,10 +173,17 @@ class BatchRunner:
         variants = variants or {}
         self.groups = [_Group(dt, members, arrays, device, variants.get(_NAME[dt], 0), dispatcher)
                        for dt, members in sorted(groups.items(), key=lambda kv: kv[0].name)]
-        # sort scratch sized before any capture: the largest group
-        need = max(g.elements for g in self.groups)
-        reserve = lib().awkward_hip_reserve_scratch
-        reserve(int(24 * max(need, 1) + 4096))
+        # scratch and the reductions' queue sized before any capture: the
+        # largest group (the sort's merge buffers and chunk queue; the
+        # reductions' long-list chunks and partials)
+        need = max(max(g.elements for g in self.groups), 1)
+        lib_ = lib()
+        lib_.awkward_hip_reserve_scratch.argtypes = [ctypes.c_int64]
+        lib_.awkward_hip_reserve_queue.argtypes = [ctypes.c_int64]
+        if lib_.awkward_hip_reserve_scratch(int(24 * need + need // 65 * 48 + (2 * need // 4096 + 4) * 48 + 8192)):
+            raise MemoryError("cannot reserve the batch runner's scratch memory")
+        if lib_.awkward_hip_reserve_queue(int((need // 8192 + need // 4097 + 4) * 96 + 8192)):
+            raise MemoryError("cannot reserve the batch runner's queue memory")
 
     def run(self):
         set_stream()

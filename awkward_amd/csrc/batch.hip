@@ -64,6 +64,17 @@ struct BatchEpilogue {
   }
 };
 
+// Per-item data and epilogue for the long-list queue's kernels.
+template <typename T, typename S>
+struct BatchSrc {
+  const BatchReduceItem* items;
+  __device__ const T* data(int i) const { return (const T*)items[i].data; }
+  __device__ BatchEpilogue<T, S> epi(int i) const {
+    const BatchReduceItem& it = items[i];
+    return BatchEpilogue<T, S>{(S*)it.out_sum, (T*)it.out_min, it.out_argmin, it.out_mask};
+  }
+};
+
 // item owning global tile `tile`: the last one with tile_begin <= tile
 template <typename Item>
 __device__ inline int find_item(const Item* items, int nitems, int64_t tile) {
@@ -82,7 +93,7 @@ template <typename T, typename S, int BLOCK, int LPT, int CAP>
 __global__ __launch_bounds__(BLOCK) void batch_reduce_kernel(const BatchReduceItem* __restrict__ items, int nitems,
                                                              int64_t total_tiles, unsigned long long* counter,
                                                              T identity, const int32_t* __restrict__ choice,
-                                                             int self) {
+                                                             int self, ReduceQueue q) {
   using Op = SumMinArgminOp<T, S>;
   __shared__ ReduceSmem<Op, BLOCK, LPT, CAP> sm;
   __shared__ int64_t s_tile;
@@ -100,11 +111,12 @@ __global__ __launch_bounds__(BLOCK) void batch_reduce_kernel(const BatchReduceIt
     __syncthreads();
     const int64_t tile = s_tile;
     if (tile >= total_tiles) break;
-    const BatchReduceItem it = items[s_item];
+    const int item = s_item;
+    const BatchReduceItem it = items[item];
     __syncthreads();  // s_tile / s_item are rewritten by the next claim
     const BatchEpilogue<T, S> epi{(S*)it.out_sum, (T*)it.out_min, it.out_argmin, it.out_mask};
     reduce_tile<Op, BLOCK, LPT, CAP>(op, epi, (const T*)it.data, it.offsets, it.n,
-                                     (tile - it.tile_begin) * (int64_t)(BLOCK * LPT), sm);
+                                     (tile - it.tile_begin) * (int64_t)(BLOCK * LPT), sm, q, item);
   }
 }
 
@@ -134,13 +146,13 @@ __global__ __launch_bounds__(sortimpl::kBlock) void batch_argsort_kernel(const B
 
 template <typename T, typename S, int BLOCK, int LPT, int CAP>
 Error launch_batch_reduce(const BatchReduceItem* items, int nitems, int64_t total_tiles, unsigned long long* counter,
-                          T identity, const int32_t* choice, int self) {
+                          T identity, const int32_t* choice, int self, const ReduceQueue& q) {
   // persistent grid: a few workgroups per CU, never more than the tiles
   int64_t grid = (int64_t)cu_count() * (BLOCK <= 64 ? 16 : BLOCK <= 128 ? 8 : 4);
   if (grid > total_tiles) grid = total_tiles;
   if (grid < 1) grid = 1;
   batch_reduce_kernel<T, S, BLOCK, LPT, CAP><<<(unsigned)grid, BLOCK, 0, stream()>>>(
-      items, nitems, total_tiles, counter, identity, choice, self);
+      items, nitems, total_tiles, counter, identity, choice, self, q);
   AKHIP_LAUNCH_CHECK("batch_reduce_kernel");
   return success();
 }
@@ -148,19 +160,64 @@ Error launch_batch_reduce(const BatchReduceItem* items, int nitems, int64_t tota
 // The ten tiles of launch_reduce (threads x lists per thread x LDS bytes).
 template <typename T, typename S>
 Error launch_batch_variant(int variant, const BatchReduceItem* it, int nitems, int64_t tiles,
-                           unsigned long long* counter, T identity, const int32_t* choice) {
+                           unsigned long long* counter, T identity, const int32_t* choice, const ReduceQueue& q) {
   switch (variant) {
-    case 1: return launch_batch_reduce<T, S, 256, 4, 8192>(it, nitems, tiles, counter, identity, choice, 1);
-    case 2: return launch_batch_reduce<T, S, 256, 2, 16384>(it, nitems, tiles, counter, identity, choice, 2);
-    case 3: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, tiles, counter, identity, choice, 3);
-    case 4: return launch_batch_reduce<T, S, 256, 1, 65536>(it, nitems, tiles, counter, identity, choice, 4);
-    case 5: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, tiles, counter, identity, choice, 5);
-    case 6: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, tiles, counter, identity, choice, 6);
-    case 7: return launch_batch_reduce<T, S, 128, 2, 16384>(it, nitems, tiles, counter, identity, choice, 7);
-    case 8: return launch_batch_reduce<T, S, 64, 4, 8192>(it, nitems, tiles, counter, identity, choice, 8);
-    case 9: return launch_batch_reduce<T, S, 128, 1, 8192>(it, nitems, tiles, counter, identity, choice, 9);
-    default: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, tiles, counter, identity, choice, 0);
+    case 1: return launch_batch_reduce<T, S, 256, 4, 8192>(it, nitems, tiles, counter, identity, choice, 1, q);
+    case 2: return launch_batch_reduce<T, S, 256, 2, 16384>(it, nitems, tiles, counter, identity, choice, 2, q);
+    case 3: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, tiles, counter, identity, choice, 3, q);
+    case 4: return launch_batch_reduce<T, S, 256, 1, 65536>(it, nitems, tiles, counter, identity, choice, 4, q);
+    case 5: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, tiles, counter, identity, choice, 5, q);
+    case 6: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, tiles, counter, identity, choice, 6, q);
+    case 7: return launch_batch_reduce<T, S, 128, 2, 16384>(it, nitems, tiles, counter, identity, choice, 7, q);
+    case 8: return launch_batch_reduce<T, S, 64, 4, 8192>(it, nitems, tiles, counter, identity, choice, 8, q);
+    case 9: return launch_batch_reduce<T, S, 128, 1, 8192>(it, nitems, tiles, counter, identity, choice, 9, q);
+    default: return launch_batch_reduce<T, S, 256, 1, 32768>(it, nitems, tiles, counter, identity, choice, 0, q);
   }
+}
+
+// The long-list queue of one batch reduction, from the queue pool: the
+// count, the chunk table, then one partial per chunk.
+template <typename T, typename S>
+bool batch_queue(int64_t total_elements, ReduceQueue* q) {
+  using Part = typename SumMinArgminOp<T, S>::Part;
+  const int64_t cap = reduce_queue_capacity(total_elements > 0 ? total_elements : 1);
+  const size_t cb = ((size_t)cap * sizeof(ReduceChunk) + 255) / 256 * 256;
+  const size_t pb = ((size_t)cap * sizeof(Part) + 255) / 256 * 256;
+  char* m = (char*)queue_pool(256 + cb + pb);
+  if (!m) return false;
+  *q = ReduceQueue{(ReduceChunk*)(m + 256), (void*)(m + 256 + cb), (unsigned long long*)m, cap};
+  return true;
+}
+
+// tiles over all the batch's lists (variant `variant`, or all ten predicated
+// on `choice`), then the queued long lists across the chip
+template <typename T, typename S>
+Error batch_reduce(const BatchReduceItem* const* tables, const int64_t* tiles, int nitems,
+                   unsigned long long* counter, T identity, int variant, const int32_t* choice,
+                   int64_t total_elements) {
+  ReduceQueue q;
+  if (!batch_queue<T, S>(total_elements, &q))
+    return failure("out of device memory for the batch reduction queue", kSliceNone, kSliceNone,
+                   "awkward_amd/csrc/batch.hip");
+  hipStream_t st = stream();
+  reduce_queue_reset<><<<1, 1, 0, st>>>(q.count);
+  for (int v = 0; v < 10; v++) {
+    const int k = choice ? v : variant;
+    const int64_t nt = choice ? tiles[v] : tiles[0];
+    if (nt > 0) {
+      Error e = launch_batch_variant<T, S>(k, choice ? tables[v] : tables[0], nitems, nt, counter, identity, choice, q);
+      if (e.str) return e;
+    }
+    if (!choice) break;
+  }
+  const SumMinArgminOp<T, S> op{identity};
+  const BatchSrc<T, S> src{tables[0]};  // every table lists the same items in the same order
+  int64_t grid = (int64_t)cu_count() * 8;
+  reduce_chunks_kernel<SumMinArgminOp<T, S>, BatchSrc<T, S>><<<(unsigned)grid, 256, 0, st>>>(op, src, q);
+  AKHIP_LAUNCH_CHECK("reduce_chunks_kernel");
+  reduce_finalize_kernel<SumMinArgminOp<T, S>, BatchSrc<T, S>><<<(unsigned)cu_count(), 256, 0, st>>>(op, src, q);
+  AKHIP_LAUNCH_CHECK("reduce_finalize_kernel");
+  return success();
 }
 
 // scratch: the long-segment queue and two (key, index) merge buffers over
@@ -218,24 +275,21 @@ Error awkward_hip_batch_reset(unsigned long long* counter) {
 // built for each variant's lists per tile; choice (device): the variant to
 // run (awkward_hip_dispatch_infer's output).  All ten launches are enqueued,
 // each leaves at once unless chosen, so no host synchronisation is needed.
+// total_elements: the batch's element count (sizes the long-list queue).
 #define BATCH_REDUCE(TN, T, S)                                                                                   \
   Error awkward_hip_batch_reduce_##TN(const void* items, int32_t nitems, int64_t total_tiles,                   \
-                                      unsigned long long* counter, T identity, int32_t variant) {              \
+                                      unsigned long long* counter, T identity, int32_t variant,                \
+                                      int64_t total_elements) {                                                \
     if (nitems <= 0 || total_tiles <= 0) return success();                                                     \
-    return launch_batch_variant<T, S>(variant, (const BatchReduceItem*)items, nitems, total_tiles, counter,    \
-                                      identity, nullptr);                                                      \
+    const BatchReduceItem* t = (const BatchReduceItem*)items;                                                  \
+    return batch_reduce<T, S>(&t, &total_tiles, nitems, counter, identity, variant, nullptr, total_elements);  \
   }                                                                                                            \
   Error awkward_hip_batch_reduce_##TN##_chosen(const void* const* tables, const int64_t* total_tiles,          \
                                                int32_t nitems, unsigned long long* counter, T identity,        \
-                                               const int32_t* choice) {                                        \
+                                               const int32_t* choice, int64_t total_elements) {                \
     if (nitems <= 0) return success();                                                                         \
-    for (int v = 0; v < 10; v++) {                                                                             \
-      if (total_tiles[v] <= 0) continue;                                                                       \
-      Error e = launch_batch_variant<T, S>(v, (const BatchReduceItem*)tables[v], nitems, total_tiles[v],       \
-                                           counter, identity, choice);                                         \
-      if (e.str) return e;                                                                                     \
-    }                                                                                                          \
-    return success();                                                                                          \
+    return batch_reduce<T, S>((const BatchReduceItem* const*)tables, total_tiles, nitems, counter, identity,   \
+                              0, choice, total_elements);                                                      \
   }
 BATCH_REDUCE(int32, int32_t, int64_t)
 BATCH_REDUCE(int64, int64_t, int64_t)

@@ -44,6 +44,9 @@ uint64_t take_error();
 // Scratch memory of at least `bytes` on the current device, valid until the
 // next call of scratch() on this device (stream-ordered reuse).
 void* scratch(size_t bytes);
+// A second such pool, for work queues that live across the kernels of one
+// entry point while its caller may hold scratch() memory.
+void* queue_pool(size_t bytes);
 // Error describing a failed HIP runtime call.
 Error hip_failure(hipError_t err, const char* what);
 // Number of compute units of the current device (cached).

@@ -474,10 +474,10 @@ Error awkward_hip_reduce_argmin_local_float32(int64_t*, int8_t*, const float*, c
 Error awkward_hip_reduce_argmin_local_float64(int64_t*, int8_t*, const double*, const int64_t*, int64_t, int);
 Error awkward_hip_batch_reset(unsigned long long* counter);
 int64_t awkward_hip_batch_reduce_tile_lists(int variant);
-Error awkward_hip_batch_reduce_int32(const void*, int32_t, int64_t, unsigned long long*, int32_t, int32_t);
-Error awkward_hip_batch_reduce_int64(const void*, int32_t, int64_t, unsigned long long*, int64_t, int32_t);
-Error awkward_hip_batch_reduce_float32(const void*, int32_t, int64_t, unsigned long long*, float, int32_t);
-Error awkward_hip_batch_reduce_float64(const void*, int32_t, int64_t, unsigned long long*, double, int32_t);
+Error awkward_hip_batch_reduce_int32(const void*, int32_t, int64_t, unsigned long long*, int32_t, int32_t, int64_t);
+Error awkward_hip_batch_reduce_int64(const void*, int32_t, int64_t, unsigned long long*, int64_t, int32_t, int64_t);
+Error awkward_hip_batch_reduce_float32(const void*, int32_t, int64_t, unsigned long long*, float, int32_t, int64_t);
+Error awkward_hip_batch_reduce_float64(const void*, int32_t, int64_t, unsigned long long*, double, int32_t, int64_t);
 
 int32_t awkward_hip_dispatch_layout(int32_t what) {
   switch (what) {
@@ -565,12 +565,13 @@ Error awkward_hip_mlp_adam(float* params, float* m, float* v, const float* grads
 
 // The exhaustive sweep: time every variant of `op` (0 sum, 1 min with mask,
 // 2 local argmin with mask, 3 the batch kernel's sum+min+argmin over this one
-// array) on one array of dtype (0 int32, 1 int64, 2 float32, 3 float64):
+// array) on one array of `length` elements, dtype (0 int32, 1 int64, 2
+// float32, 3 float64):
 // one untimed launch, then `reps` launches between two events on the current
 // stream.  ms_out (host) receives kVariantsN milliseconds per launch.  Uses
 // the scratch pool for the outputs; synchronises.
 Error awkward_hip_dispatch_time_variants(int32_t op, int32_t dtype, const void* data, const int64_t* offsets,
-                                         int64_t n, int32_t reps, float* ms_out) {
+                                         int64_t n, int64_t length, int32_t reps, float* ms_out) {
   const char* file = "awkward_amd/csrc/learned.hip";
   if (op < 0 || op > 3 || dtype < 0 || dtype > 3 || reps < 1 || n < 0)
     return failure("invalid timing request", kSliceNone, kSliceNone, file);
@@ -619,10 +620,10 @@ Error awkward_hip_dispatch_time_variants(int32_t op, int32_t dtype, const void* 
         Error e = awkward_hip_batch_reset(counter);
         if (e.str) return e;
         switch (dtype) {
-          case 0: return awkward_hip_batch_reduce_int32(s, 1, tiles, counter, INT32_MAX, v);
-          case 1: return awkward_hip_batch_reduce_int64(s, 1, tiles, counter, INT64_MAX, v);
-          case 2: return awkward_hip_batch_reduce_float32(s, 1, tiles, counter, INFINITY, v);
-          default: return awkward_hip_batch_reduce_float64(s, 1, tiles, counter, (double)INFINITY, v);
+          case 0: return awkward_hip_batch_reduce_int32(s, 1, tiles, counter, INT32_MAX, v, length);
+          case 1: return awkward_hip_batch_reduce_int64(s, 1, tiles, counter, INT64_MAX, v, length);
+          case 2: return awkward_hip_batch_reduce_float32(s, 1, tiles, counter, INFINITY, v, length);
+          default: return awkward_hip_batch_reduce_float64(s, 1, tiles, counter, (double)INFINITY, v, length);
         }
       }
     }

@@ -16,6 +16,11 @@
 //     resolve to the first occurrence exactly as the serial loop does; only
 //     float sums of lists longer than kShortLen in heavy tiles change their
 //     summation order (within the stated tolerance).
+//   * a caller may pass a ReduceQueue: lists longer than kWaveLen then go to
+//     it in kReduceChunk-element chunks instead of to the workgroup, and
+//     reduce_chunks_kernel (every workgroup of the chip, one chunk each) +
+//     reduce_finalize_kernel (chunk partials combined in order) finish them
+//     -- so a few very long lists no longer serialise on a few workgroups.
 //
 // Ops (struct with the members used below):
 //   In, Out, Part
@@ -246,6 +251,113 @@ struct SumMinArgminOp {
   __device__ Part shfl_xor(Part p, int m) const { return Part{shfl_xor_any(p.sum, m), shfl_xor_vi(p.m, m)}; }
 };
 
+// ---- the long-list queue ----------------------------------------------------------
+constexpr int64_t kReduceChunk = 8192;
+
+struct ReduceChunk {
+  int64_t bin;     // output position; -1 marks an unused slot
+  int64_t gb, ge;  // the list's element range
+  int64_t first;   // queue position of the list's first chunk (its partials start there)
+  int32_t chunk, nchunks;
+  int32_t item, pad;  // item: which array of a batch
+};
+
+struct ReduceQueue {
+  ReduceChunk* chunks;
+  void* partials;  // one Part per queue position
+  unsigned long long* count;
+  int64_t capacity;  // 0: no queue
+};
+
+// Queue capacity for `elements` elements: every queued list exceeds kWaveLen.
+inline int64_t reduce_queue_capacity(int64_t elements) {
+  return (elements + kReduceChunk - 1) / kReduceChunk + elements / (kWaveLen + 1) + 2;
+}
+
+// Called by every lane of a wave; lanes with `want` try to queue their list
+// (one atomic per wave).  If the queue is full the wave's lists stay with
+// the workgroup (false) and its reserved slots are marked unused.
+__device__ inline bool queue_reduce(const ReduceQueue& q, bool want, int64_t bin, int64_t gb, int64_t ge,
+                                    int item) {
+  if (!__any(want)) return false;
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long nch = want ? (unsigned long long)((ge - gb + kReduceChunk - 1) / kReduceChunk) : 0ull;
+  const unsigned long long incl = wave_inclusive_scan(nch);
+  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(q.count, total);
+  base = __shfl(base, 0, kWave);
+  const int64_t first = (int64_t)(base + incl - nch);
+  if ((int64_t)(base + total) > q.capacity) {
+    for (int64_t c = 0; c < (int64_t)nch; c++)
+      if (first + c < q.capacity) q.chunks[first + c].bin = -1;
+    return false;
+  }
+  for (int64_t c = 0; c < (int64_t)nch; c++)
+    q.chunks[first + c] = ReduceChunk{bin, gb, ge, first, (int32_t)c, (int32_t)nch, item, 0};
+  return want;
+}
+
+template <int Dummy = 0>
+__global__ void reduce_queue_reset(unsigned long long* count) { *count = 0ull; }
+
+// One queued chunk per workgroup iteration: a Part over its elements
+// (four interleaved accumulators per thread, then the waves in order).
+template <class Op, class Src>
+__global__ __launch_bounds__(256) void reduce_chunks_kernel(Op op, Src src, ReduceQueue q) {
+  using Part = typename Op::Part;
+  using In = typename Op::In;
+  constexpr int NW = 256 / kWave;
+  __shared__ Part s_part[NW];
+  int64_t nc = (int64_t)*q.count;
+  if (nc > q.capacity) nc = q.capacity;
+  Part* parts = (Part*)q.partials;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const ReduceChunk ch = q.chunks[c];
+    if (ch.bin < 0) continue;  // workgroup-uniform
+    const In* __restrict__ from = src.data(ch.item);
+    const int64_t i0 = ch.gb + (int64_t)ch.chunk * kReduceChunk;
+    const int64_t i1 = (i0 + kReduceChunk) < ch.ge ? i0 + kReduceChunk : ch.ge;
+    Part p[4] = {op.init(), op.init(), op.init(), op.init()};
+    for (int64_t i = i0 + tid; i < i1; i += 4 * 256) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t k = i + u * 256;
+        if (k < i1) p[u] = op.combine(p[u], op.elem(from[k], k));
+      }
+    }
+    Part r = op.combine(op.combine(p[0], p[1]), op.combine(p[2], p[3]));
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) r = op.combine(r, op.shfl_xor(r, d));
+    if (lane == 0) s_part[wave] = r;
+    __syncthreads();
+    if (tid == 0) {
+      Part t = s_part[0];
+      for (int w = 1; w < NW; w++) t = op.combine(t, s_part[w]);
+      parts[c] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Each queued list: its chunks' partials combined in chunk order, stored by
+// the item's epilogue.
+template <class Op, class Src>
+__global__ void reduce_finalize_kernel(Op op, Src src, ReduceQueue q) {
+  using Part = typename Op::Part;
+  int64_t nc = (int64_t)*q.count;
+  if (nc > q.capacity) nc = q.capacity;
+  const Part* parts = (const Part*)q.partials;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const ReduceChunk ch = q.chunks[c];
+    if (ch.bin < 0 || ch.chunk != 0) continue;
+    Part p = parts[ch.first];
+    for (int k = 1; k < ch.nchunks; k++) p = op.combine(p, parts[ch.first + k]);
+    src.epi(ch.item).store(ch.bin, op.finalize(p, src.data(ch.item), ch.gb, ch.ge), ch.gb, ch.ge);
+  }
+}
+
 // ---- the tile -----------------------------------------------------------------
 // Epilogues own the stores: store(bin, value, gb, ge) with the list's global
 // element range [gb, ge).  PlainEpilogue writes the value; the fused
@@ -271,12 +383,15 @@ struct ReduceSmem {
 };
 
 // One tile: the lists [L0, L0 + BLOCK * LPT) of `offsets` (clipped to
-// outlength); thread t owns lists t + j * BLOCK.  Ends with a barrier, so a
-// persistent caller may reuse `sm` for its next tile at once.
+// outlength); thread t owns lists t + j * BLOCK.  With a queue (capacity >
+// 0), lists longer than kWaveLen are queued (item: the batch item they
+// belong to).  Ends with a barrier, so a persistent caller may reuse `sm`
+// for its next tile at once.
 template <class Op, int BLOCK, int LPT, int CAP_BYTES, class Epi>
 __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename Op::In* __restrict__ fromptr,
                                    const int64_t* __restrict__ offsets, int64_t outlength, int64_t L0,
-                                   ReduceSmem<Op, BLOCK, LPT, CAP_BYTES>& sm) {
+                                   ReduceSmem<Op, BLOCK, LPT, CAP_BYTES>& sm, const ReduceQueue& q = ReduceQueue{},
+                                   int item = 0) {
   using In = typename Op::In;
   using Out = typename Op::Out;
   using Part = typename Op::Part;
@@ -359,7 +474,9 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
       for (int d = kWave / 2; d > 0; d >>= 1) p = op.combine(p, op.shfl_xor(p, d));
       if (lane == 0) epi.store(L0 + owner, op.finalize(p, fromptr, lb, le), lb, le);
     }
-    const uint64_t lng = __ballot(t < nl && len > kWaveLen);
+    const bool want = t < nl && len > kWaveLen;
+    const bool queued = q.capacity > 0 && queue_reduce(q, want, L0 + t, gb, ge, item);
+    const uint64_t lng = __ballot(want && !queued);
     if (lane == 0) sm.lng[j][wave] = lng;
   }
   __syncthreads();

@@ -23,6 +23,8 @@ struct DeviceState {
   uint64_t* host_word = nullptr;   // pinned host mirror
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  void* queue = nullptr;  // second pool: work queues of multi-kernel sequences
+  size_t queue_bytes = 0;
   int cus = 0;
 };
 
@@ -99,27 +101,37 @@ uint64_t take_error() {
   return key;
 }
 
-void* scratch(size_t bytes) {
-  DeviceState& s = state();
+// Grow-only device buffer: at least `bytes`, contents not preserved on growth.
+void* grow(void** buf, size_t* have, size_t bytes) {
   if (bytes < 256) bytes = 256;
-  if (s.scratch_bytes < bytes) {
+  if (*have < bytes) {
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (s.scratch_bytes < bytes) {
-      size_t want = s.scratch_bytes ? s.scratch_bytes : (size_t)1 << 20;
+    if (*have < bytes) {
+      size_t want = *have ? *have : (size_t)1 << 20;
       while (want < bytes) want *= 2;
-      if (s.scratch) {
+      if (*buf) {
         (void)hipDeviceSynchronize();  // in-flight kernels may still use it
-        (void)hipFree(s.scratch);
-        s.scratch = nullptr;
-        s.scratch_bytes = 0;
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *have = 0;
       }
       void* p = nullptr;
       if (hipMalloc(&p, want) != hipSuccess) return nullptr;
-      s.scratch = p;
-      s.scratch_bytes = want;
+      *buf = p;
+      *have = want;
     }
   }
-  return s.scratch;
+  return *buf;
+}
+
+void* scratch(size_t bytes) {
+  DeviceState& s = state();
+  return grow(&s.scratch, &s.scratch_bytes, bytes);
+}
+
+void* queue_pool(size_t bytes) {
+  DeviceState& s = state();
+  return grow(&s.queue, &s.queue_bytes, bytes);
 }
 
 Error hip_failure(hipError_t err, const char* what) {
@@ -169,6 +181,11 @@ const char* awkward_hip_version(void) { return "awkward_hip 0.1 (gfx950)"; }
 // which no allocation may happen).  Returns 0 on success.
 int awkward_hip_reserve_scratch(int64_t bytes) {
   return scratch((size_t)bytes) != nullptr ? 0 : 1;
+}
+
+// The same for the work-queue pool (the batch reductions' long-list queue).
+int awkward_hip_reserve_queue(int64_t bytes) {
+  return queue_pool((size_t)bytes) != nullptr ? 0 : 1;
 }
 
 }  // extern "C"

@@ -326,8 +326,9 @@ def time_variants(op, offsets, data, n, reps):
     """ms per launch of each of the ten tiles (the exhaustive sweep)."""
     out = (ctypes.c_float * N_VARIANTS)()
     set_stream()
-    _check(_fn("awkward_hip_dispatch_time_variants", [I32, I32, P, P, I64, I32, P])(
-        op, DTYPE_CODE[data.dtype], data.data_ptr(), offsets.data_ptr(), n, reps, ctypes.addressof(out)))
+    _check(_fn("awkward_hip_dispatch_time_variants", [I32, I32, P, P, I64, I64, I32, P])(
+        op, DTYPE_CODE[data.dtype], data.data_ptr(), offsets.data_ptr(), n, data.shape[0], reps,
+        ctypes.addressof(out)))
     return np.array(out[:], dtype=np.float64)
 
 

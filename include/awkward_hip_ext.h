@@ -23,6 +23,8 @@ extern "C" {
 /* Grow the per-device scratch pool to at least `bytes` now (before capturing
    a hipGraph whose kernels use scratch).  0 on success. */
 int awkward_hip_reserve_scratch(int64_t bytes);
+/* The same for the work-queue pool (the batch reductions' long-list queue). */
+int awkward_hip_reserve_queue(int64_t bytes);
 /* Number of reduce tile variants (threads x lists per thread x LDS stage). */
 int awkward_hip_reduce_variant_count(void);
 
@@ -148,21 +150,24 @@ struct Error awkward_hip_pair_expr(float* out, const float* pt1, const float* et
    reduce_<T>_chosen: all ten tiles enqueued, each predicated on the device
    int choice[0] (awkward_hip_dispatch_infer); tables / total_tiles are HOST
    arrays of ten.
+   total_elements: the batch's element count; lists longer than 4096
+   elements are reduced in 8192-element chunks by every workgroup (queue
+   from the work-queue pool).
    argsort: stable ascending/descending argsort per list, local indices. */
 struct Error awkward_hip_batch_reset(unsigned long long* counter);
 int64_t awkward_hip_batch_reduce_tile_lists(int variant);
 int64_t awkward_hip_batch_sort_tile_lists(void);
-struct Error awkward_hip_batch_reduce_int32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int32_t identity, int32_t variant);
-struct Error awkward_hip_batch_reduce_int32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int32_t identity, const int32_t* choice);
+struct Error awkward_hip_batch_reduce_int32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int32_t identity, int32_t variant, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_int32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int32_t identity, const int32_t* choice, int64_t total_elements);
 struct Error awkward_hip_batch_argsort_int32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
-struct Error awkward_hip_batch_reduce_int64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int64_t identity, int32_t variant);
-struct Error awkward_hip_batch_reduce_int64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int64_t identity, const int32_t* choice);
+struct Error awkward_hip_batch_reduce_int64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, int64_t identity, int32_t variant, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_int64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, int64_t identity, const int32_t* choice, int64_t total_elements);
 struct Error awkward_hip_batch_argsort_int64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
-struct Error awkward_hip_batch_reduce_float32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, float identity, int32_t variant);
-struct Error awkward_hip_batch_reduce_float32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, float identity, const int32_t* choice);
+struct Error awkward_hip_batch_reduce_float32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, float identity, int32_t variant, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_float32_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, float identity, const int32_t* choice, int64_t total_elements);
 struct Error awkward_hip_batch_argsort_float32(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
-struct Error awkward_hip_batch_reduce_float64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, double identity, int32_t variant);
-struct Error awkward_hip_batch_reduce_float64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, double identity, const int32_t* choice);
+struct Error awkward_hip_batch_reduce_float64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, double identity, int32_t variant, int64_t total_elements);
+struct Error awkward_hip_batch_reduce_float64_chosen(const void* const* tables, const int64_t* total_tiles, int32_t nitems, unsigned long long* counter, double identity, const int32_t* choice, int64_t total_elements);
 struct Error awkward_hip_batch_argsort_float64(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* counter, bool ascending, int64_t total_elements);
 
 /* ---- learned tile dispatcher ---------------------------------------------------
@@ -188,7 +193,7 @@ struct Error awkward_hip_dispatch_infer(const unsigned long long* acc, const int
 struct Error awkward_hip_mlp_forward(const float* X, int64_t rows, const float* params, float* Y, int32_t* inside);
 struct Error awkward_hip_mlp_train_step(const float* X, const float* T, const float* ow, int64_t R, int64_t pa, int64_t pb, int64_t row0, int64_t bsz, const float* params, float* grads, float* losses);
 struct Error awkward_hip_mlp_adam(float* params, float* m, float* v, const float* grads, const float* losses, int32_t nparts, float lr, float beta1, float beta2, float eps, int64_t step, float* loss_acc);
-struct Error awkward_hip_dispatch_time_variants(int32_t op, int32_t dtype, const void* data, const int64_t* offsets, int64_t n, int32_t reps, float* ms_out);
+struct Error awkward_hip_dispatch_time_variants(int32_t op, int32_t dtype, const void* data, const int64_t* offsets, int64_t n, int64_t length, int32_t reps, float* ms_out);
 
 #ifdef __cplusplus
 }
