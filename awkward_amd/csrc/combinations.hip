@@ -7,11 +7,14 @@
 //
 // The CPU enumerates each list's n-tuples recursively.  Here the tuples are
 // decoded independently per output slot: a scan gives every list its first
-// slot, each slot finds its list by binary search over those offsets and
-// unranks its local rank in lexicographic order (combinatorial number system,
-// one binary search per tuple position over closed-form counts).  Slots are
+// slot, each slot finds its list by binary search over those offsets (kept
+// in LDS per tile) and unranks its local rank in lexicographic order (closed
+// forms for pairs and triples, else one binary search per tuple position
+// over closed-form counts).  Slots are
 // spread over threads so that all n carries are written with coalesced
 // stores, which is what bounds this kernel (8 * n bytes per tuple).
+
+#include <type_traits>
 
 #include "errors.h"
 #include "scan.h"
@@ -175,10 +178,41 @@ __global__ void zero_two(int64_t* a, int64_t* b) {
 }
 
 constexpr int kDecodeBlock = 256;
-constexpr int kDecodeSlots = 4;  // slots per thread per tile
+constexpr int kWindow = 2048;  // lists whose offsets a tile may keep in LDS
+
+// Tuples per thread: pairs and triples run 8 slots per thread, general n 4.
+template <int N>
+constexpr int decode_run() {
+  return (N == 2 || N == 3) ? 8 : 4;
+}
+
+// The next tuple of a list in lexicographic order: combinations of [0, s)
+// (x strictly increasing, x_q <= s - n + q) or, with replacement,
+// multisets (x non-decreasing, x_q <= s - 1).  False when x was the last.
+template <int NQ, typename X>
+__device__ inline bool next_tuple(X* x, X s, int64_t n, bool replacement) {
+  int q = -1;
+#pragma unroll
+  for (int k = 0; k < NQ; k++)
+    if (k < n && x[k] < (replacement ? s - 1 : s - (X)n + k)) q = k;
+  if (q < 0) return false;
+  x[q]++;
+#pragma unroll
+  for (int k = 1; k < NQ; k++)
+    if (k > q && k < n) x[k] = replacement ? x[q] : x[k - 1] + 1;
+  return true;
+}
 
 // Persistent decode over P = offsets[length] slots (read on the device, so
-// no host synchronisation is needed between the scan and the decode).
+// no host synchronisation is needed between the scan and the decode).  Each
+// workgroup decodes a contiguous run of tiles: the offsets window starting
+// at the previous tile's last list (sized from the mean tuples per list) is
+// loaded in one coalesced pass and the lists are found by binary search in
+// LDS (a tile whose lists run past the window searches global memory).  A
+// thread unranks the first of its RUN consecutive slots only and steps to
+// the others with next_tuple (moving on to the next list with tuples when
+// one is exhausted); each carry then goes through LDS so that the stores are
+// coalesced.
 template <typename C, int N>
 __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __restrict__ tocarry,
                                                               int64_t* __restrict__ toindex,
@@ -186,80 +220,122 @@ __global__ __launch_bounds__(kDecodeBlock) void decode_kernel(int64_t* const* __
                                                               const C* __restrict__ starts,
                                                               const C* __restrict__ stops, int64_t length,
                                                               int64_t n, bool replacement) {
-  constexpr int64_t TILE = (int64_t)kDecodeBlock * kDecodeSlots;
-  constexpr int kCache = 2048;  // offsets of the tile's lists kept in LDS
+  constexpr int RUN = decode_run<N>();
+  constexpr int NQ = N > 0 ? N : kMaxN;
+  constexpr int64_t TILE = (int64_t)kDecodeBlock * RUN;
   __shared__ int64_t* s_out[kMaxN];
-  __shared__ int64_t s_off[kCache + 1];
-  __shared__ int64_t s_range[2];
+  __shared__ int64_t s_off[kWindow + 1];
+  __shared__ int64_t s_hint;
   if (threadIdx.x < n) s_out[threadIdx.x] = tocarry[threadIdx.x];
   const int64_t P = offsets[length];
   if (blockIdx.x == 0 && threadIdx.x < n) toindex[threadIdx.x] = P;
-  // each workgroup decodes a contiguous run of tiles, so the next tile's
-  // lists are found by galloping forward from the previous tile's last list
   const int64_t ntiles = (P + TILE - 1) / TILE;
   const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
   const int64_t t_begin = (int64_t)blockIdx.x * per;
   const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
-  int64_t hint = 0;  // a list with offsets[hint] <= the next tile's first slot
+  if (t_begin >= t_end) return;  // workgroup-uniform, before any barrier
+  // window length: about twice the lists a tile spans on average
+  int64_t want = (int64_t)(2.0 * (double)TILE * (double)length / (double)(P > 0 ? P : 1)) + 64;
+  if (want > kWindow + 1) want = kWindow + 1;
+  // largest l >= lo with offsets[l] <= slot: the list holding the slot (a
+  // list without tuples repeats an offset)
+  auto search = [&](int64_t lo, int64_t slot) {
+    int64_t hi = length;  // offsets[length] = P > slot
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (offsets[mid] <= slot) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  if (threadIdx.x == 0) s_hint = search(0, t_begin * TILE);
+  __syncthreads();
+  int64_t hint = s_hint;
   for (int64_t tile = t_begin; tile < t_end; tile++) {
     const int64_t p0 = tile * TILE;
     const int64_t p1 = (p0 + TILE < P ? p0 + TILE : P) - 1;  // last slot of the tile
-    if (threadIdx.x < 2) {
-      const int64_t target = threadIdx.x == 0 ? p0 : p1;
-      // largest l with offsets[l] <= target, from hint: gallop, then bisect
-      int64_t lo = hint, step = 1;
-      while (lo + step < length && offsets[lo + step] <= target) {
-        lo += step;
-        step <<= 1;
-      }
-      int64_t hi = lo + step < length ? lo + step : length;  // offsets[hi] > target or hi == length
+    const int64_t wn = want < length + 1 - hint ? want : length + 1 - hint;  // offsets[hint .. hint + wn)
+    for (int64_t i = threadIdx.x; i < wn; i += kDecodeBlock) s_off[i] = offsets[hint + i];
+    __syncthreads();
+    const bool cached = s_off[wn - 1] > p1;  // the window holds every list of the tile
+    auto off_at = [&](int64_t l) { return cached ? s_off[l - hint] : offsets[l]; };
+    auto local = [&](int64_t p) {  // s_off[lo] <= p < s_off[lo + 1]
+      int64_t lo = 0, hi = wn - 1;
       while (hi - lo > 1) {
         const int64_t mid = (lo + hi) >> 1;
-        if (offsets[mid] <= target) lo = mid;
+        if (s_off[mid] <= p) lo = mid;
         else hi = mid;
       }
-      s_range[threadIdx.x] = lo;
-    }
-    __syncthreads();
-    const int64_t l0 = s_range[0], l1 = s_range[1];
-    hint = l1;
-    const int64_t nl = l1 - l0 + 1;
-    const bool cached = nl <= kCache;
-    if (cached)
-      for (int64_t t = threadIdx.x; t <= nl; t += kDecodeBlock) s_off[t] = offsets[l0 + t];
-    __syncthreads();
+      return lo;
+    };
+    // a run: the list start of each slot and the tuple's positions in its
+    // list (32-bit for pairs and triples: a list whose pairs fit in memory
+    // has fewer than 2^31 elements)
+    using X = typename std::conditional<(N == 2 || N == 3), int32_t, int64_t>::type;
+    int64_t run_start[RUN];
+    X out[RUN][NQ];
+    const int64_t p = p0 + (int64_t)threadIdx.x * RUN;
+    if (p <= p1) {
+      int64_t l = cached ? hint + local(p) : search(hint, p);
+      int64_t end = off_at(l + 1);  // first slot of the next list
+      int64_t start = (int64_t)starts[l];
+      X s = (X)((int64_t)stops[l] - start);
+      X x[NQ];
+      {
+        int64_t xr[kMaxN];
+        unrank<N>(p - off_at(l), (int64_t)s, n, replacement, xr);
 #pragma unroll
-    for (int u = 0; u < kDecodeSlots; u++) {
-      const int64_t p = p0 + (int64_t)u * kDecodeBlock + threadIdx.x;
-      if (p > p1) break;
-      // list with offsets[l] <= p < offsets[l+1] (lists without tuples repeat
-      // an offset, so this is the last list starting at or before p)
-      int64_t lo = 0, hi = nl, base;
-      if (cached) {
-        while (hi - lo > 1) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (s_off[mid] <= p) lo = mid;
-          else hi = mid;
-        }
-        base = s_off[lo];
-      } else {
-        while (hi - lo > 1) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (offsets[l0 + mid] <= p) lo = mid;
-          else hi = mid;
-        }
-        base = offsets[l0 + lo];
+        for (int q = 0; q < NQ; q++) x[q] = (X)xr[q];
       }
-      const int64_t l = l0 + lo;
-      const int64_t start = (int64_t)starts[l];
-      const int64_t s = (int64_t)stops[l] - start;
-      int64_t x[kMaxN];
-      unrank<N>(p - base, s, n, replacement, x);
 #pragma unroll
-      for (int q = 0; q < (N > 0 ? N : kMaxN); q++)
-        if (q < n) s_out[q][p] = start + x[q];
+      for (int u = 0; u < RUN; u++) {
+        const int64_t slot = p + u;
+        if (slot > p1) break;
+        if (slot == end) {  // the next list with tuples starts here
+          do {
+            l++;
+            end = off_at(l + 1);
+          } while (end <= slot);
+          start = (int64_t)starts[l];
+          s = (X)((int64_t)stops[l] - start);
+#pragma unroll
+          for (int q = 0; q < NQ; q++) x[q] = replacement ? 0 : (X)q;
+        } else if (u > 0) {
+          next_tuple<NQ, X>(x, s, n, replacement);
+        }
+        run_start[u] = start;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[u][q] = x[q];
+      }
     }
-    __syncthreads();  // s_off and s_range are rewritten by the next tile
+    // each carry: the run's RUN consecutive slots are RUN * 8 contiguous
+    // bytes, written as 16-byte stores when aligned (whole lines fill in L2
+    // from neighbouring threads)
+    if (p <= p1) {
+      const bool full = p + RUN - 1 <= p1;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (q >= n) break;
+        int64_t* dst = s_out[q] + p;
+        if (full && (((uintptr_t)dst) & 15) == 0) {
+#pragma unroll
+          for (int u = 0; u < RUN; u += 2) {
+            longlong2 v;
+            v.x = run_start[u] + (int64_t)out[u][q];
+            v.y = run_start[u + 1] + (int64_t)out[u + 1][q];
+            *reinterpret_cast<longlong2*>(dst + u) = v;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < RUN; u++)
+            if (p + u <= p1) dst[u] = run_start[u] + (int64_t)out[u][q];
+        }
+      }
+    }
+    if (threadIdx.x == 0) s_hint = cached ? hint + local(p1) : search(hint, p1);
+    __syncthreads();  // s_off and s_hint are rewritten by the next tile
+    hint = s_hint;
+    __syncthreads();
   }
 }
 
@@ -279,16 +355,36 @@ struct OffsetsBody {
   }
 };
 
+// Workgroups of `kernel` that fit on one CU at once (cached per kernel).
+template <typename K>
+int resident_per_cu(K kernel) {
+  static int cached = 0;
+  if (cached == 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kDecodeBlock, 0) != hipSuccess || b < 1) b = 4;
+    cached = b;
+  }
+  return cached;
+}
+
+template <typename C, int N>
+void decode(int64_t** tocarry, int64_t* toindex, const int64_t* offsets, const C* starts, const C* stops,
+            int64_t length, int64_t n, bool replacement, int64_t grid_hint, hipStream_t st) {
+  // persistent: exactly the workgroups the chip holds at once, so no
+  // workgroup waits for another one's whole run of tiles
+  int64_t grid = (int64_t)cu_count() * resident_per_cu(decode_kernel<C, N>);
+  if (grid_hint > 0 && grid_hint < grid) grid = grid_hint;
+  decode_kernel<C, N><<<(unsigned)grid, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n,
+                                                               replacement);
+}
+
 template <typename C>
 Error launch_decode(int64_t** tocarry, int64_t* toindex, const int64_t* offsets, const C* starts, const C* stops,
                     int64_t length, int64_t n, bool replacement, int64_t grid_hint) {
-  int64_t grid = (int64_t)cu_count() * 16;
-  if (grid_hint > 0 && grid_hint < grid) grid = grid_hint;
-  const unsigned g = (unsigned)grid;
   hipStream_t st = stream();
-  if (n == 2) decode_kernel<C, 2><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
-  else if (n == 3) decode_kernel<C, 3><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
-  else decode_kernel<C, 0><<<g, kDecodeBlock, 0, st>>>(tocarry, toindex, offsets, starts, stops, length, n, replacement);
+  if (n == 2) decode<C, 2>(tocarry, toindex, offsets, starts, stops, length, n, replacement, grid_hint, st);
+  else if (n == 3) decode<C, 3>(tocarry, toindex, offsets, starts, stops, length, n, replacement, grid_hint, st);
+  else decode<C, 0>(tocarry, toindex, offsets, starts, stops, length, n, replacement, grid_hint, st);
   AKHIP_LAUNCH_CHECK("combinations decode_kernel");
   return success();
 }
@@ -361,7 +457,7 @@ Error awkward_hip_combinations_from_offsets(int64_t** tocarry, int64_t* toindex,
                                             bool replacement, int64_t totallen) {
   if (n <= 0 || length <= 0) return success();
   if (n > kMaxN) return failure("combinations with n > 8 are not supported on the hip backend", kSliceNone, kSliceNone, kFile);
-  const int64_t tiles = ceil_div(totallen > 0 ? totallen : 1, (int64_t)kDecodeBlock * kDecodeSlots);
+  const int64_t tiles = ceil_div(totallen > 0 ? totallen : 1, (int64_t)kDecodeBlock * decode_run<0>());
   return launch_decode<int64_t>(tocarry, toindex, tooffsets, starts, stops, length, n, replacement, tiles);
 }
 

@@ -312,57 +312,132 @@ __device__ inline float pair_expr(float pt1, float eta1, float phi1, float pt2, 
   return b * (c - d);
 }
 
-constexpr int kBlock = 256;
-constexpr int kPairStage = 2048;
+constexpr int kPairEvents = 128;  // events (and threads) per workgroup
+constexpr int kPairStage = 1024;  // particles staged per workgroup
+constexpr int kPairCap = 2048;    // pair values kept per workgroup
 
-__global__ __launch_bounds__(kBlock) void pair_max_kernel(const int64_t* __restrict__ offsets, int64_t n,
-                                                          const float* __restrict__ pt, const float* __restrict__ eta,
-                                                          const float* __restrict__ phi, float* __restrict__ outmax,
-                                                          int64_t* __restrict__ outarg, int8_t* __restrict__ outmask,
-                                                          float identity) {
+// Workgroup of 128 events.  When the events' particles and pair values fit
+// the LDS tiles, the pairs are spread evenly over the threads -- each takes
+// a contiguous run, finds the first pair's event by binary search over the
+// pair-count prefix and steps through the rest in combinations order -- and
+// the values go to LDS; then every thread reduces its own event's values in
+// order.  Otherwise each thread enumerates its event's pairs itself.
+__global__ __launch_bounds__(kPairEvents) void pair_max_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                                                               const float* __restrict__ pt,
+                                                               const float* __restrict__ eta,
+                                                               const float* __restrict__ phi,
+                                                               float* __restrict__ outmax,
+                                                               int64_t* __restrict__ outarg,
+                                                               int8_t* __restrict__ outmask, float identity) {
   __shared__ float s_pt[kPairStage], s_eta[kPairStage], s_phi[kPairStage];
-  __shared__ int64_t s_off[kBlock + 1];
+  __shared__ float s_val[kPairCap];
+  __shared__ int64_t s_off[kPairEvents + 1];
+  __shared__ int s_pbeg[kPairEvents + 1];  // pair-count prefix (valid when staged)
+  __shared__ int s_wsum[kPairEvents / kWave];
   const int tid = threadIdx.x;
-  const int64_t L0 = (int64_t)blockIdx.x * kBlock;
-  const int nl = (int)((n - L0) < kBlock ? (n - L0) : kBlock);
-  for (int t = tid; t <= nl; t += kBlock) s_off[t] = offsets[L0 + t];
+  const int64_t L0 = (int64_t)blockIdx.x * kPairEvents;
+  const int nl = (int)((n - L0) < kPairEvents ? (n - L0) : kPairEvents);
+  for (int t = tid; t <= nl; t += kPairEvents) s_off[t] = offsets[L0 + t];
   __syncthreads();
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
   const bool staged = span <= kPairStage;
   if (staged) {
-    for (int64_t q = tid; q < span; q += kBlock) {
+    for (int64_t q = tid; q < span; q += kPairEvents) {
       s_pt[q] = pt[E0 + q];
       s_eta[q] = eta[E0 + q];
       s_phi[q] = phi[E0 + q];
     }
   }
+  // pairs of this thread's event, and their exclusive prefix over the
+  // workgroup (staged: every event has at most kPairStage particles, so the
+  // total stays below 128 * C(1024, 2) < 2^31)
+  const int m_own = tid < nl ? (int)(s_off[tid + 1] - s_off[tid]) : 0;
+  const int c_own = staged ? m_own * (m_own - 1) / 2 : 0;
+  const int incl = wave_inclusive_scan(c_own);
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  if (lane == kWave - 1) s_wsum[wave] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wave; w++) wbase += s_wsum[w];
+  int total = 0;
+  for (int w = 0; w < kPairEvents / kWave; w++) total += s_wsum[w];
+  s_pbeg[tid] = wbase + incl - c_own;
+  if (tid == 0) s_pbeg[kPairEvents] = total;
+  __syncthreads();
+  const bool pairwise = staged && total <= kPairCap;
+  if (pairwise && total > 0) {
+    const int per = (total + kPairEvents - 1) / kPairEvents;
+    int q = tid * per;
+    const int qend = q + per < total ? q + per : total;
+    if (q < qend) {
+      // the event holding pair q: last e with s_pbeg[e] <= q
+      int lo = 0, hi = nl;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pbeg[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+      int e = lo;
+      while (s_pbeg[e + 1] <= q) e++;  // events without pairs share a prefix value
+      int m = (int)(s_off[e + 1] - s_off[e]);
+      int r = q - s_pbeg[e];
+      int i = 0;
+      while (r >= m - 1 - i) {
+        r -= m - 1 - i;
+        i++;
+      }
+      int j = i + 1 + r;
+      int a = (int)(s_off[e] - E0);
+      for (; q < qend; q++) {
+        s_val[q] = pair_expr(s_pt[a + i], s_eta[a + i], s_phi[a + i], s_pt[a + j], s_eta[a + j], s_phi[a + j]);
+        if (++j == m) {
+          ++i;
+          j = i + 1;
+        }
+        if (i >= m - 1 && q + 1 < qend) {  // on to the next event with pairs
+          do {
+            e++;
+          } while (s_pbeg[e + 1] == s_pbeg[e]);
+          m = (int)(s_off[e + 1] - s_off[e]);
+          a = (int)(s_off[e] - E0);
+          i = 0;
+          j = 1;
+        }
+      }
+    }
+  }
   __syncthreads();
   if (tid >= nl) return;
   const int64_t ev = L0 + tid;
-  const int64_t a = s_off[tid], b = s_off[tid + 1];
-  // LDS is indexed with in-range local positions only (a pointer formed
-  // below an LDS array wraps in the 32-bit LDS address space)
-  const int64_t base = staged ? E0 : 0;
-  const float* P = staged ? s_pt : pt;
-  const float* H = staged ? s_eta : eta;
-  const float* F = staged ? s_phi : phi;
   // ak.max: best = identity; best = v > best ? v : best   (awkward_reduce_max.cpp)
   // ak.argmax: first pair seeds, strict improvement only  (awkward_reduce_argmax.cpp)
   float best = identity, argval = 0.0f;
   int64_t arg = -1, r = 0;
-  for (int64_t i = a - base; i < b - base; i++) {
-    const float pi = P[i], ei = H[i], fi = F[i];
-    for (int64_t j = i + 1; j < b - base; j++, r++) {
-      const float v = pair_expr(pi, ei, fi, P[j], H[j], F[j]);
-      best = (v > best) ? v : best;
-      if (r == 0) {
-        arg = 0;
-        argval = v;
-      } else if (v > argval) {
-        argval = v;
-        arg = r;
-      }
+  auto take = [&](float v) {
+    best = (v > best) ? v : best;
+    if (r == 0) {
+      arg = 0;
+      argval = v;
+    } else if (v > argval) {
+      argval = v;
+      arg = r;
+    }
+    r++;
+  };
+  if (pairwise) {
+    for (int q = s_pbeg[tid]; q < s_pbeg[tid + 1]; q++) take(s_val[q]);
+  } else {
+    const int64_t a = s_off[tid], b = s_off[tid + 1];
+    // LDS is indexed with in-range local positions only (a pointer formed
+    // below an LDS array wraps in the 32-bit LDS address space)
+    const int64_t base = staged ? E0 : 0;
+    const float* P = staged ? s_pt : pt;
+    const float* H = staged ? s_eta : eta;
+    const float* F = staged ? s_phi : phi;
+    for (int64_t i = a - base; i < b - base; i++) {
+      const float pi = P[i], ei = H[i], fi = F[i];
+      for (int64_t j = i + 1; j < b - base; j++) take(pair_expr(pi, ei, fi, P[j], H[j], F[j]));
     }
   }
   outmax[ev] = best;
@@ -463,8 +538,8 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_pa
 Error awkward_hip_pair_max(const int64_t* offsets, int64_t n, const float* pt, const float* eta, const float* phi,
                            float* out_max, int64_t* out_argmax, int8_t* out_mask, float identity) {
   if (n <= 0) return success();
-  pair_max_kernel<<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream()>>>(offsets, n, pt, eta, phi, out_max,
-                                                                          out_argmax, out_mask, identity);
+  pair_max_kernel<<<(unsigned)ceil_div(n, kPairEvents), kPairEvents, 0, stream()>>>(offsets, n, pt, eta, phi, out_max,
+                                                                                    out_argmax, out_mask, identity);
   AKHIP_LAUNCH_CHECK("pair_max_kernel");
   return success();
 }

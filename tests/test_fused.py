@@ -58,14 +58,18 @@ def test_event_pipeline_long_events(cpu, hip):
     assert same_bits(flat(ak.sum(sel.pt, axis=-1))[0], out["sum"].cpu().numpy())
 
 
-def test_pair_max_matches_elementwise_form(hip):
+@pytest.mark.parametrize("lam,clip", [(4, 30), (14, 60), (0.7, 3)])
+def test_pair_max_matches_elementwise_form(hip, lam, clip):
     """pair_max == ak.max/argmax over the materialised pairs with the same
-    device expression (bit-exact), and the torch-ufunc form (tolerance)."""
+    device expression (bit-exact), and the torch-ufunc form (tolerance).
+    Poisson(4): the pairs of a workgroup fit its LDS tile (pair-parallel);
+    Poisson(14): they do not (per-event enumeration); Poisson(0.7): most
+    events have no pairs."""
     import torch
 
     from awkward_amd.fused import PairMax, pair_expr
 
-    off, fields = events(7, 20000, 4, 30)
+    off, fields = events(7, 20000, lam, clip)
     ev = ak.from_offsets(off, fields, hip)
     pairs = ak.combinations(ev, 2)
     p0, p1 = pairs["0"], pairs["1"]

@@ -268,6 +268,30 @@ def test_combinations(c, n, replacement):
     assert same_bits(h[1], d[1])
 
 
+@pytest.mark.parametrize("n", [2, 3])
+@pytest.mark.parametrize("replacement", [False, True])
+def test_combinations_skewed(n, replacement):
+    """Runs of thousands of lists without tuples between a few very long
+    lists: tiles whose lists overflow the decode's offsets window."""
+    rng = np.random.default_rng(25)
+    counts = np.zeros(30000, dtype=np.int64)
+    counts[rng.integers(0, 30000, 20)] = rng.integers(40, 120, 20)
+    counts[:5000] = rng.integers(0, n, 5000)  # no tuples without replacement
+    off = offsets_from_counts(counts)
+    nl = len(counts)
+    starts, stops = off[:-1], off[1:]
+    h, d, _, _ = run_both("awkward_ListArray64_combinations_length_64",
+                          [np.zeros(1, np.int64), np.zeros(nl + 1, np.int64), n, replacement, starts, stops, nl])
+    assert same_bits(h[0], d[0]) and same_bits(h[1], d[1])
+    total = int(h[0][0])
+    carry = [np.zeros(total, np.int64) for _ in range(n)]
+    h, d, _, _ = run_both("awkward_ListArray64_combinations_64",
+                          [carry, np.zeros(n, np.int64), np.zeros(n, np.int64), n, replacement, starts, stops, nl])
+    for k in range(n):
+        assert same_bits(h[0][k], d[0][k]), k
+    assert same_bits(h[1], d[1])
+
+
 SORT_TYPES = ["bool"] + ALL_TYPES
 
 
