@@ -313,8 +313,8 @@ __device__ inline float pair_expr(float pt1, float eta1, float phi1, float pt2, 
 }
 
 constexpr int kPairEvents = 128;  // events (and threads) per workgroup
-constexpr int kPairStage = 1024;  // particles staged per workgroup
-constexpr int kPairCap = 2048;    // pair values kept per workgroup
+constexpr int kPairStage = 768;   // particles staged per workgroup (~1.5x a workgroup at Poisson(4))
+constexpr int kPairCap = 1536;    // pair values kept per workgroup (small tiles: more workgroups per CU)
 
 // Workgroup of 128 events.  When the events' particles and pair values fit
 // the LDS tiles, the pairs are spread evenly over the threads -- each takes
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(kPairEvents) void pair_max_kernel(const int64_t* __
   }
   // pairs of this thread's event, and their exclusive prefix over the
   // workgroup (staged: every event has at most kPairStage particles, so the
-  // total stays below 128 * C(1024, 2) < 2^31)
+  // total stays below 128 * C(768, 2) < 2^31)
   const int m_own = tid < nl ? (int)(s_off[tid + 1] - s_off[tid]) : 0;
   const int c_own = staged ? m_own * (m_own - 1) / 2 : 0;
   const int incl = wave_inclusive_scan(c_own);

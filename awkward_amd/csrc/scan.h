@@ -7,11 +7,15 @@
 //
 //   * each workgroup claims a tile id from an atomic counter, so every tile
 //     it waits on has already started (no dependence on dispatch order);
-//   * the tile's items are counted (Body::count), scanned in registers and
+//   * the tile's items are counted (Body::count) striped over the threads
+//     (item u * BLOCK + t), so the loads coalesce, and transposed through LDS
+//     so that each thread then scans IPT consecutive items in registers;
 //     across the wave with shuffles, then across waves through LDS;
 //   * the tile publishes its aggregate, and wave 0 looks back over up to 64
 //     predecessors at a time until it meets a published inclusive prefix;
-//   * Body::emit(i, exclusive_prefix, count) writes the outputs.
+//   * the exclusive prefixes go back through LDS and Body::emit(i,
+//     exclusive_prefix, count) runs striped again, so neighbouring lanes
+//     write neighbouring outputs.
 //
 // Status words pack a 2-bit flag with a 62-bit non-negative value, so one
 // 64-bit atomic carries both and no other memory needs ordering: the loads
@@ -56,9 +60,12 @@ template <int BLOCK, int IPT, class Body>
 __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScratch s, int64_t n,
                                                               uint64_t* errword, uint32_t inv) {
   constexpr int NW = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
   __shared__ int64_t s_wave[NW];
   __shared__ int64_t s_prefix;
   __shared__ unsigned s_tile;
+  __shared__ int64_t s_v[TILE + TILE / 32];  // one pad slot per 32 values
+  auto pad = [](int p) { return p + (p >> 5); };
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
@@ -66,15 +73,23 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
   if (tid == 0) s_tile = atomicAdd(s.counter, 1u);
   __syncthreads();
   const int64_t tile = s_tile;
-  const int64_t base = tile * (int64_t)(BLOCK * IPT) + (int64_t)tid * IPT;
+  const int64_t tbase = tile * (int64_t)TILE;
 
+  // counts striped (coalesced loads), then each thread's IPT consecutive items
   int64_t c[IPT];
-  int64_t total = 0;
 #pragma unroll
   for (int u = 0; u < IPT; u++) {
-    const int64_t i = base + u;
+    const int64_t i = tbase + (int64_t)u * BLOCK + tid;
     c[u] = (i < n) ? body.count(i) : 0;
-    total += c[u];
+    s_v[pad(u * BLOCK + tid)] = c[u];
+  }
+  __syncthreads();
+  int64_t b[IPT];
+  int64_t total = 0;
+#pragma unroll
+  for (int v = 0; v < IPT; v++) {
+    b[v] = s_v[pad(tid * IPT + v)];
+    total += b[v];
   }
   const int64_t incl = wave_inclusive_scan(total);
   if (lane == kWave - 1) s_wave[wave] = incl;
@@ -132,12 +147,18 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
     if (lane == 0) s_prefix = prefix;
   }
   __syncthreads();
+  // exclusive prefixes of the consecutive items back to LDS, emits striped
   int64_t run = s_prefix + wave_off + incl - total;
 #pragma unroll
+  for (int v = 0; v < IPT; v++) {
+    s_v[pad(tid * IPT + v)] = run;
+    run += b[v];
+  }
+  __syncthreads();
+#pragma unroll
   for (int u = 0; u < IPT; u++) {
-    const int64_t i = base + u;
-    if (i < n) body.emit(i, run, c[u]);
-    run += c[u];
+    const int64_t i = tbase + (int64_t)u * BLOCK + tid;
+    if (i < n) body.emit(i, s_v[pad(u * BLOCK + tid)], c[u]);
   }
 }
 
