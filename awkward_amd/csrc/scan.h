@@ -56,6 +56,55 @@ __global__ void scan_prep_kernel(ScanScratch s, int64_t ntiles) {
   if (i < ntiles) s.status[i] = 0;
 }
 
+// Called by every lane of one wave of tile `tile`'s workgroup: publishes the
+// tile's aggregate and returns (to every lane) the sum of all earlier tiles,
+// publishing this tile's inclusive prefix in turn.
+__device__ inline int64_t lookback_prefix(ScanScratch s, int64_t tile, int64_t block_total, uint64_t* errword,
+                                          uint32_t inv) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int64_t prefix = 0;
+  if (tile == 0) {
+    if (lane == 0)
+      __hip_atomic_store(&s.status[0], kScanPrefix | (uint64_t)block_total, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&s.status[tile], kScanAgg | (uint64_t)block_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int64_t look = tile - 1;
+  bool done = false;
+  while (!done) {
+    const int64_t idx = look - lane;
+    uint64_t w = kScanPrefix;  // lanes before tile 0 read as an empty prefix
+    if (idx >= 0) {
+      uint32_t spins = 0;
+      do {
+        w = __hip_atomic_load(&s.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((w & kScanFlags) != 0) break;
+        __builtin_amdgcn_s_sleep(1);
+      } while (++spins < (1u << 26));
+      if ((w & kScanFlags) == 0) {
+        record_error(errword, inv, tile, kScanTimeoutCode);
+        w = kScanPrefix;
+      }
+    }
+    const uint64_t prefix_lanes = __ballot((w & kScanFlags) == kScanPrefix);
+    int64_t v = (int64_t)(w & kScanValue);
+    if (prefix_lanes != 0) {
+      const int first = __builtin_ctzll(prefix_lanes);
+      v = (lane <= first) ? v : 0;
+      done = true;
+    } else {
+      look -= kWave;
+    }
+    prefix += wave_sum(v);
+  }
+  if (lane == 0)
+    __hip_atomic_store(&s.status[tile], kScanPrefix | (uint64_t)(prefix + block_total), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
 template <int BLOCK, int IPT, class Body>
 __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScratch s, int64_t n,
                                                               uint64_t* errword, uint32_t inv) {
@@ -103,47 +152,7 @@ __global__ __launch_bounds__(BLOCK) void lookback_scan_kernel(Body body, ScanScr
   }
 
   if (wave == 0) {
-    int64_t prefix = 0;
-    if (tile == 0) {
-      if (lane == 0)
-        __hip_atomic_store(&s.status[0], kScanPrefix | (uint64_t)block_total, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&s.status[tile], kScanAgg | (uint64_t)block_total, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      int64_t look = tile - 1;
-      bool done = false;
-      while (!done) {
-        const int64_t idx = look - lane;
-        uint64_t w = kScanPrefix;  // lanes before tile 0 read as an empty prefix
-        if (idx >= 0) {
-          uint32_t spins = 0;
-          do {
-            w = __hip_atomic_load(&s.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((w & kScanFlags) != 0) break;
-            __builtin_amdgcn_s_sleep(1);
-          } while (++spins < (1u << 26));
-          if ((w & kScanFlags) == 0) {
-            record_error(errword, inv, tile, kScanTimeoutCode);
-            w = kScanPrefix;
-          }
-        }
-        const uint64_t prefix_lanes = __ballot((w & kScanFlags) == kScanPrefix);
-        int64_t v = (int64_t)(w & kScanValue);
-        if (prefix_lanes != 0) {
-          const int first = __builtin_ctzll(prefix_lanes);
-          v = (lane <= first) ? v : 0;
-          done = true;
-        } else {
-          look -= kWave;
-        }
-        prefix += wave_sum(v);
-      }
-      if (lane == 0)
-        __hip_atomic_store(&s.status[tile], kScanPrefix | (uint64_t)(prefix + block_total),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const int64_t prefix = lookback_prefix(s, tile, block_total, errword, inv);
     if (lane == 0) s_prefix = prefix;
   }
   __syncthreads();
