@@ -326,6 +326,60 @@ def traffic_probe(args):
           flush=True)
 
 
+def roofline_probe(args):
+    """The program the per-kernel counter runs profile: the 1 GiB copy the
+    counters are calibrated on, then every workload's kernels once, eagerly
+    (graph replays are not profiled per dispatch): config 1's reductions
+    (plain and tuned), the event pipeline (plain, tuned and fused), config
+    3's combinations (plain and fused) and a config-4 batch (sum/min/argmin
+    and argsort)."""
+    import numpy as np
+    import torch
+
+    import awkward_amd as ak
+    from awkward_amd.backend import HipBackend
+    from awkward_amd.batch import BatchRunner
+    from awkward_amd.fused import Combinations, EventPipeline, PairMax
+
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    plain, tuned = HipBackend(fused=False), HipBackend(fused=True)
+    rng = np.random.default_rng(1)
+    counts = rng.poisson(10, 1_000_000)
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    x = rng.standard_normal(int(off[-1]))
+    for be in (plain, tuned):
+        arr = ak.from_offsets(off, x, be)
+        for fn in (ak.sum, ak.max, ak.argmax, ak.count):
+            fn(arr, axis=-1)
+    torch.cuda.synchronize()
+    off2, fields = make_events(args.events)
+    for be in (plain, tuned):
+        pipeline(ak, ak.from_offsets(off2, fields, be))
+    ev = ak.from_offsets(off2, fields, tuned)
+    EventPipeline(ev, [("pt", ">", 25.0), ("eta", "abs<", 2.4)], slice(1, None), "pt",
+                  outputs=("num", "sum", "argmax")).run()
+    torch.cuda.synchronize()
+    del ev
+    off3, f3 = make_events(20_000_000, seed=3, lam=4, clip=30)
+    f3 = {k: f3[k] for k in ("pt", "eta", "phi")}
+    ev3 = ak.from_offsets(off3, f3, tuned)
+    config3_step(ak, np, ev3)
+    PairMax(ev3).run()
+    Combinations(ev3, 3).run()
+    torch.cuda.synchronize()
+    del ev3
+    batch = make_batch(max_exp=4.5)
+    dev = [(torch.from_numpy(o).cuda(), torch.from_numpy(v).cuda()) for o, v in batch]
+    BatchRunner(dev, variants={n: 0 for n in ("int32", "int64", "float32", "float64")}).run()
+    torch.cuda.synchronize()
+    print("roofline probe done", flush=True)
+
+
 def cpu_baseline(ak, off, fields, sample_events):
     import numpy as np
 
@@ -376,6 +430,7 @@ def main():
     ap.add_argument("--no-config4", action="store_true")
     ap.add_argument("--no-config5", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help="only the program profiled by the counter runs")
+    ap.add_argument("--roofline-probe", action="store_true", help="every workload's kernels once, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each op is independent per "
@@ -384,6 +439,9 @@ def main():
 
     if args.traffic_probe:
         traffic_probe(args)
+        return
+    if args.roofline_probe:
+        roofline_probe(args)
         return
 
     import torch
