@@ -155,19 +155,21 @@ __device__ void sort_wave(void* out, const T* from, int64_t a, int len, bool asc
 __device__ inline int pad16(int i) { return i + (i >> 4); }
 constexpr int kCapPad = kCap + kCap / 16;
 
-// Sorts n <= kCap pairs (order_key(from[a + i]), idx_base + i) of the
-// workgroup: each thread sorts 16 in registers, then log2(np / 16) merge
-// stages in LDS, each thread finding its 16 outputs' merge-path split by
-// binary search (np = n rounded up to a power of two, at least 16).  The
-// sorted pairs end in s_key / s_idx at positions pad16(0 .. n-1).
-template <typename T>
-__device__ void block_sort(uint64_t* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
-                           uint32_t idx_base) {
+// Sorts n <= THREADS * 16 pairs (order_key(from[a + i]), idx_base + i) with
+// THREADS threads (t = 0 .. THREADS-1): each sorts 16 in registers, then
+// log2(np / 16) merge stages in LDS, each thread finding its 16 outputs'
+// merge-path split by binary search (np = n rounded up to a power of two, at
+// least 16).  Every thread of the workgroup calls it with the same np_sync
+// >= np (the merge stages and their barriers run up to np_sync), so several
+// groups of threads may sort different segments side by side.  The sorted
+// pairs end in s_key / s_idx at positions pad16(0 .. n-1).
+template <typename T, int THREADS>
+__device__ void block_sort_part(uint64_t* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
+                                uint32_t idx_base, int t, int np_sync) {
   constexpr int PER = 16;
   int np = PER;
   while (np < n) np <<= 1;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < np; i += kBlock) {
+  for (int i = t; i < np; i += THREADS) {
     const int p = pad16(i);
     if (i < n) {
       s_key[p] = order_key<T>(from[a + i], asc);
@@ -178,10 +180,10 @@ __device__ void block_sort(uint64_t* s_key, uint32_t* s_idx, const T* from, int6
     }
   }
   __syncthreads();
-  const bool active = tid * PER < np;
+  const bool active = t * PER < np;
   uint64_t k[PER];
   uint32_t x[PER];
-  const int o = tid * PER;  // this thread's first position
+  const int o = t * PER;  // this thread's first position
   if (active) {
 #pragma unroll
     for (int u = 0; u < PER; u++) {
@@ -196,8 +198,9 @@ __device__ void block_sort(uint64_t* s_key, uint32_t* s_idx, const T* from, int6
     }
   }
   __syncthreads();
-  for (int r = PER; r < np; r <<= 1) {
-    if (active) {
+  for (int r = PER; r < np_sync; r <<= 1) {
+    const bool merge = active && r < np;
+    if (merge) {
       const int ps = o / (2 * r) * (2 * r);
       const int d = o - ps;
       const int A0 = ps, B0 = ps + r;
@@ -225,7 +228,7 @@ __device__ void block_sort(uint64_t* s_key, uint32_t* s_idx, const T* from, int6
       }
     }
     __syncthreads();
-    if (active) {
+    if (merge) {
 #pragma unroll
       for (int u = 0; u < PER; u++) {
         s_key[pad16(o + u)] = k[u];
@@ -263,26 +266,48 @@ struct LongChunk {
   int64_t chunk;     // chunk index within the segment
 };
 
-// The queue: capacity >= sum over queued segments of ceil(len / kCap), at
-// most elements / (kWave + 1) + 2 * elements / kCap since every queued
-// segment exceeds kWave elements.
-inline int64_t long_queue_capacity(int64_t elements) {
-  return elements / (kWave + 1) + 2 * ((elements + kCap - 1) / kCap) + 2;
+// Segments of kWave < len <= kSmall elements go to a queue of their own and
+// are sorted a wave each (four side by side per workgroup).
+constexpr int kSmall = kWave * 16;
+
+// Capacities: the small queue holds at most elements / (kWave + 1) segments;
+// the chunk queue sum over longer segments of ceil(len / kCap), at most
+// elements / kCap + elements / (kSmall + 1) + 1.
+inline int64_t small_queue_capacity(int64_t elements) { return elements / (kWave + 1) + 2; }
+inline int64_t chunk_queue_capacity(int64_t elements) {
+  return (elements + kCap - 1) / kCap + elements / (kSmall + 1) + 2;
 }
 
 struct LongQueue {
   LongChunk* chunks;
   unsigned long long* count;   // chunks queued
-  unsigned long long* maxlen;  // longest queued segment
+  unsigned long long* maxlen;  // longest segment in the chunk queue
   int64_t capacity;
+  LongChunk* small;
+  unsigned long long* small_count;
+  int64_t small_capacity;
 };
 
 // Called by every lane of a wave: the lanes with want set append their
-// segment's chunks; one atomic per wave reserves the room.
+// segment -- one entry in the small queue, or its chunks in the chunk queue;
+// one atomic per wave and queue reserves the room.
 __device__ inline void queue_segments(const LongQueue& q, bool want, void* out, const void* from, int64_t a,
                                       int64_t len, int64_t buf) {
   if (!__any(want)) return;
   const int lane = threadIdx.x & (kWave - 1);
+  const bool small = want && len <= kSmall;
+  if (__any(small)) {
+    const unsigned long long one = small ? 1ull : 0ull;
+    const unsigned long long sincl = wave_inclusive_scan(one);
+    const unsigned long long stotal = __shfl(sincl, kWave - 1, kWave);
+    unsigned long long sbase = 0;
+    if (lane == 0) sbase = atomicAdd(q.small_count, stotal);
+    sbase = __shfl(sbase, 0, kWave);
+    const int64_t at = (int64_t)(sbase + sincl - one);
+    if (small && at < q.small_capacity) q.small[at] = LongChunk{out, from, a, len, buf, 0};
+  }
+  want = want && !small;
+  if (!__any(want)) return;
   const unsigned long long nch = want ? (unsigned long long)((len + kCap - 1) / kCap) : 0ull;
   const unsigned long long incl = wave_inclusive_scan(nch);
   const unsigned long long total = __shfl(incl, kWave - 1, kWave);
@@ -336,27 +361,57 @@ __device__ void sort_tile(void* out, const T* __restrict__ from, int64_t length,
 // ---- queued long segments -------------------------------------------------------------
 // (a template so every translation unit may instantiate it)
 template <int Dummy = 0>
-__global__ void long_queue_reset(unsigned long long* count, unsigned long long* maxlen) {
+__global__ void long_queue_reset(unsigned long long* count, unsigned long long* maxlen,
+                                 unsigned long long* small_count) {
   *count = 0ull;
   *maxlen = 0ull;
+  *small_count = 0ull;
 }
 
-// Every chunk sorted in LDS: a segment of one chunk straight to its output,
-// longer ones into merge buffer 0 (segment-relative positions).
+// Small segments four at a time, a wave each, in quarters of the LDS tile;
+// then every chunk of the longer ones with the whole workgroup: a segment of
+// one chunk straight to its output, longer ones into merge buffer 0
+// (segment-relative positions).
 template <typename T, bool ARG>
-__global__ __launch_bounds__(kBlock) void long_runs_kernel(const LongChunk* __restrict__ chunks,
-                                                           const unsigned long long* __restrict__ count,
-                                                           int64_t capacity, bool asc, uint64_t* __restrict__ gk,
+__global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc, uint64_t* __restrict__ gk,
                                                            uint32_t* __restrict__ gi) {
+  constexpr int NW = kBlock / kWave;
+  constexpr int QUARTER = kCapPad / NW;  // pad16(kSmall) fits
   __shared__ uint64_t s_key[kCapPad];
   __shared__ uint32_t s_idx[kCapPad];
-  int64_t nc = (int64_t)*count;
-  if (nc > capacity) nc = capacity;
+  __shared__ int s_np[NW];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int64_t ns = (int64_t)*q.small_count;
+  if (ns > q.small_capacity) ns = q.small_capacity;
+  for (int64_t g = blockIdx.x; g * NW < ns; g += gridDim.x) {
+    const int64_t e = g * NW + wave;
+    LongChunk ch{};
+    int n = 0;
+    if (e < ns) {
+      ch = q.small[e];
+      n = (int)ch.len;
+    }
+    int np = 16;
+    while (np < n) np <<= 1;
+    if (lane == 0) s_np[wave] = np;
+    __syncthreads();
+    int np_sync = s_np[0];
+    for (int w = 1; w < NW; w++) np_sync = s_np[w] > np_sync ? s_np[w] : np_sync;
+    block_sort_part<T, kWave>(s_key + wave * QUARTER, s_idx + wave * QUARTER, (const T*)ch.from, ch.a, n, asc, 0,
+                              lane, np_sync);
+    for (int i = lane; i < n; i += kWave)
+      store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, s_idx[wave * QUARTER + pad16(i)]);
+    __syncthreads();
+  }
+  int64_t nc = (int64_t)*q.count;
+  if (nc > q.capacity) nc = q.capacity;
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-    const LongChunk ch = chunks[c];
+    const LongChunk ch = q.chunks[c];
     const int64_t c0 = ch.chunk * kCap;
     const int n = (int)((ch.len - c0) < kCap ? (ch.len - c0) : kCap);
-    block_sort<T>(s_key, s_idx, (const T*)ch.from, ch.a + c0, n, asc, (uint32_t)c0);
+    int np = 16;
+    while (np < n) np <<= 1;
+    block_sort_part<T, kBlock>(s_key, s_idx, (const T*)ch.from, ch.a + c0, n, asc, (uint32_t)c0, threadIdx.x, np);
     if (ch.len <= kCap) {
       for (int i = threadIdx.x; i < n; i += kBlock)
         store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, s_idx[pad16(i)]);
@@ -519,9 +574,9 @@ inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc
                                    uint64_t* gk1, uint32_t* gi1, hipStream_t st) {
   if (max_len <= kWave) return hipSuccess;
   int64_t grid = (int64_t)cu_count() * 8;
-  if (grid > q.capacity) grid = q.capacity;
+  if (grid > q.capacity + q.small_capacity) grid = q.capacity + q.small_capacity;
   if (grid < 1) grid = 1;
-  long_runs_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.capacity, asc, gk0, gi0);
+  long_runs_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q, asc, gk0, gi0);
   uint64_t *sk = gk0, *dk = gk1;
   uint32_t *si = gi0, *di = gi1;
   for (int64_t w = kCap; w < max_len; w <<= 1) {

@@ -54,16 +54,21 @@ struct SortScratch {
 inline bool sort_scratch(int64_t length, SortScratch* out) {
   const size_t kbytes = ((size_t)length * 8 + 255) / 256 * 256;
   const size_t ibytes = ((size_t)length * 4 + 255) / 256 * 256;
-  const int64_t capacity = long_queue_capacity(length);
+  const int64_t capacity = chunk_queue_capacity(length);
+  const int64_t small_capacity = small_queue_capacity(length);
   const size_t qbytes = ((size_t)capacity * sizeof(LongChunk) + 255) / 256 * 256;
-  char* s = (char*)scratch(512 + qbytes + 2 * (kbytes + ibytes));
+  const size_t sbytes = ((size_t)small_capacity * sizeof(LongChunk) + 255) / 256 * 256;
+  char* s = (char*)scratch(512 + qbytes + sbytes + 2 * (kbytes + ibytes));
   if (!s) return false;
   out->head = s;
   out->q.count = (unsigned long long*)(s + 256);
   out->q.maxlen = (unsigned long long*)(s + 256 + 8);
+  out->q.small_count = (unsigned long long*)(s + 256 + 16);
   out->q.chunks = (LongChunk*)(s + 512);
   out->q.capacity = capacity;
-  char* b = s + 512 + qbytes;
+  out->q.small = (LongChunk*)(s + 512 + qbytes);
+  out->q.small_capacity = small_capacity;
+  char* b = s + 512 + qbytes + sbytes;
   out->gk0 = (uint64_t*)b;
   out->gk1 = (uint64_t*)(b + kbytes);
   out->gi0 = (uint32_t*)(b + 2 * kbytes);
@@ -82,7 +87,7 @@ Error segmented_sort(void* out, const T* from, int64_t length, const U* offsets,
     AKHIP_LAUNCH_CHECK("uncovered_kernel");
     return success();
   }
-  long_queue_reset<><<<1, 1, 0, st>>>(sc.q.count, sc.q.maxlen);
+  long_queue_reset<><<<1, 1, 0, st>>>(sc.q.count, sc.q.maxlen, sc.q.small_count);
   segmented_sort_kernel<T, U, ARG><<<(unsigned)ceil_div(nseg, kBlock), kBlock, 0, st>>>(out, from, length, offsets,
                                                                                        nseg, asc, sc.q);
   AKHIP_LAUNCH_CHECK("segmented_sort_kernel");
