@@ -200,6 +200,30 @@ def pair_expr(p0, p1):
     return out
 
 
+class Concurrent:
+    """Runs independent op sequences side by side: each on its own stream
+    (default priority) forked from the current one, joined back before
+    returning.  Under capture the branches become parallel branches of the
+    hipGraph, so a latency-bound kernel overlaps a bandwidth-bound one.
+    The sequences must not share outputs."""
+
+    def __init__(self, *fns):
+        self.fns = fns
+        self.streams = [torch.cuda.Stream() for _ in fns[1:]]
+
+    def __call__(self):
+        cur = torch.cuda.current_stream()
+        results = [None] * len(self.fns)
+        for st, fn, k in zip(self.streams, self.fns[1:], range(1, len(self.fns))):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                results[k] = fn()
+        results[0] = self.fns[0]()
+        for st in self.streams:
+            cur.wait_stream(st)
+        return tuple(results)
+
+
 class Graph:
     """Captures fn() (which must not synchronise or allocate through the
     library) into a hipGraph and replays it."""

@@ -88,6 +88,33 @@ def test_pair_max_matches_elementwise_form(hip, lam, clip):
     assert np.allclose(u, expr.cpu().numpy(), rtol=2e-6, atol=0)
 
 
+def test_concurrent_branches_in_a_graph(hip):
+    """pair_max and the triples' combinations as two graph branches on their
+    own streams: every replay equals the sequential eager results."""
+    import torch
+
+    from awkward_amd.fused import Combinations, Concurrent, Graph, PairMax
+
+    off, fields = events(12, 40000, 4, 30)
+    ev = ak.from_offsets(off, fields, hip)
+    pm = PairMax(ev)
+    tri = Combinations(ev, 3)
+    ref_pm = {k: v.clone() for k, v in pm.run().items()}
+    ref_tri = [c.clone() for c in tri.run()]
+    torch.cuda.synchronize()
+    g = Graph(Concurrent(tri.run, pm.run))
+    for _ in range(3):
+        for v in list(pm.out.values()) + tri.carry:
+            v.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in ref_pm:
+            assert torch.equal(pm.out[k], ref_pm[k])
+        for a, b in zip(tri.carry, ref_tri):
+            assert torch.equal(a, b)
+
+
 def test_graph_replays_match_eager(hip):
     import torch
 
