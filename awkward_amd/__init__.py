@@ -7,6 +7,8 @@ The kernels are hand-written HIP for gfx950 in ``libawkward_hip.so``
 layer that issues the reference's call sequences for ``sum``, ``max``,
 ``min``, ``argmax``, ``argmin``, ``count``, ``num``, ``combinations``,
 jagged and range ``__getitem__``, ``sort`` and ``argsort`` at axis=-1.
+Buffers enter and leave without copies through DLPack (``from_dlpack``)
+and the Form/length/container triple (``to_buffers`` / ``from_buffers``).
 """
 
 from awkward_amd.highlevel import (  # noqa: F401
@@ -26,3 +28,4 @@ from awkward_amd.highlevel import (  # noqa: F401
     sum,
     to_backend,
 )
+from awkward_amd.interop import DLPackDevice, from_buffers, from_dlpack, to_buffers  # noqa: F401
