@@ -292,6 +292,33 @@ def test_combinations_skewed(n, replacement):
     assert same_bits(h[1], d[1])
 
 
+@pytest.mark.parametrize("n", [2, 3])
+@pytest.mark.parametrize("replacement", [False, True])
+def test_combinations_table_edges(n, replacement):
+    """List sizes around the decode's colex-table limits (pairs of up to 64
+    items, triples of up to 32, counting the n - 1 extra items of
+    multisets), on a ListArray whose lists are scattered over the content
+    (starts not equal to the previous stops)."""
+    rng = np.random.default_rng(26)
+    edges = [0, 1, 2, 3, 29, 30, 31, 32, 33, 34, 61, 62, 63, 64, 65, 66]
+    sizes = np.array(edges * 40 + list(rng.poisson(5, 3000)), dtype=np.int64)
+    rng.shuffle(sizes)
+    gaps = rng.integers(0, 7, len(sizes))
+    starts = np.cumsum(np.concatenate([[0], (sizes + gaps)[:-1]])).astype(np.int64)
+    stops = starts + sizes
+    nl = len(sizes)
+    h, d, _, _ = run_both("awkward_ListArray64_combinations_length_64",
+                          [np.zeros(1, np.int64), np.zeros(nl + 1, np.int64), n, replacement, starts, stops, nl])
+    assert same_bits(h[0], d[0]) and same_bits(h[1], d[1])
+    total = int(h[0][0])
+    carry = [np.zeros(total, np.int64) for _ in range(n)]
+    h, d, _, _ = run_both("awkward_ListArray64_combinations_64",
+                          [carry, np.zeros(n, np.int64), np.zeros(n, np.int64), n, replacement, starts, stops, nl])
+    for k in range(n):
+        assert same_bits(h[0][k], d[0][k]), k
+    assert same_bits(h[1], d[1])
+
+
 SORT_TYPES = ["bool"] + ALL_TYPES
 
 
