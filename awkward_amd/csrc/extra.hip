@@ -63,6 +63,20 @@ __global__ void sub_kernel(int64_t* __restrict__ out, const int64_t* __restrict_
 
 __global__ void zero1(int64_t* p) { *p = 0; }
 
+// A plain streaming read (16-byte loads, one per thread): the device's own
+// read rate for a buffer, the ceiling a reduction over it is measured
+// against.  Each loaded word is compared with a key known only at run time,
+// so the compiler cannot drop the loads; the host passes a key with bits
+// that no folded word has, so `sink` is never written.
+__global__ __launch_bounds__(256) void stream_read_kernel(const uint4* __restrict__ p, int64_t n16, uint64_t key,
+                                                          uint64_t* __restrict__ sink) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n16) return;
+  const uint4 v = p[i];
+  const uint64_t x = ((uint64_t)(v.x ^ v.z) << 32) | (uint64_t)(v.y ^ v.w);
+  if (x == key) sink[0] = x;
+}
+
 }  // namespace
 }  // namespace akhip
 
@@ -97,6 +111,19 @@ Error awkward_hip_bool_to_index(int64_t* toindex, int64_t* tooffsets, const bool
   if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
   BoolToIndexBody body{toindex, tooffsets, mask, offsets, 0};
   return launch_lookback_scan<256, 8>(body, n, 0, s);
+}
+
+// Measurement: reads the first nbytes / 16 * 16 bytes of `data` (16-byte
+// aligned); `sink` (8 bytes) is written only if a 16-byte word folds to one
+// fixed 64-bit pattern.
+Error awkward_hip_stream_read(const void* data, int64_t nbytes, uint64_t* sink) {
+  const int64_t n16 = nbytes / 16;
+  if (n16 <= 0) return success();
+  if (((uintptr_t)data & 15) != 0) return failure("stream_read needs 16-byte aligned data", kSliceNone, kSliceNone, kFile);
+  stream_read_kernel<<<(unsigned)ceil_div(n16, (int64_t)256), 256, 0, stream()>>>((const uint4*)data, n16,
+                                                                                     0x0123456789abcdefull, sink);
+  AKHIP_LAUNCH_CHECK("stream_read_kernel");
+  return success();
 }
 
 Error awkward_hip_subtract_i64(int64_t* out, const int64_t* a, const int64_t* b, int64_t n) {

@@ -2,7 +2,9 @@
 
 ``time_variants`` times every compiled variant of one reduction on the
 device (median of ``reps`` launches, each after an L2/MALL scrub so the
-inputs come from HBM).  ``Autotuner`` keeps the fastest variant per
+inputs come from HBM).  The scrub only reads (a sum over 1 GiB): a scrub
+that writes leaves up to 256 MiB of dirty lines in the caches, whose
+write-back would be charged to the timed kernel.  ``Autotuner`` keeps the fastest variant per
 (op, dtype, list-count bucket, mean-length bucket) key in a JSON table;
 ``install()`` makes the hip backend's reducers consult it (falling back to
 the heuristic of ``awkward_amd.dispatch`` for keys it has not measured).
@@ -44,7 +46,7 @@ def time_variants(op, out, data, offsets, n, mask=None, identity=None, reps=5, s
             raise RuntimeError(err.str.decode())
         samples = []
         for _ in range(reps):
-            scrub.add_(1.0)
+            scrub.sum()
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record()
             call(v)

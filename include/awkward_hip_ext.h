@@ -116,6 +116,11 @@ struct Error awkward_hip_take_4(void* out, const void* data, const int64_t* inde
 struct Error awkward_hip_take_8(void* out, const void* data, const int64_t* index, int64_t n);
 struct Error awkward_hip_bool_to_index(int64_t* toindex, int64_t* tooffsets, const bool* mask, const int64_t* offsets, int64_t n);
 struct Error awkward_hip_subtract_i64(int64_t* out, const int64_t* a, const int64_t* b, int64_t n);
+/* Measurement only: a plain streaming read of nbytes (rounded down to 16)
+   of 16-byte-aligned `data`, the read-rate ceiling a reduction over the same
+   buffer is compared with; `sink` (one uint64) is written only if a 16-byte
+   word folds (x ^ z, y ^ w) to one fixed 64-bit pattern. */
+struct Error awkward_hip_stream_read(const void* data, int64_t nbytes, uint64_t* sink);
 
 /* ---- combinations without a host round trip ---------------------------------
    awkward_ListArray_combinations for lists [starts, stops) whose output
