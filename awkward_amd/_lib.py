@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "lib", "libawkward_hip.so")
+# AWKWARD_HIP_LIBRARY names another build of the same library (A/B timing of
+# kernel variants); the in-tree build is the default.
+LIBPATH = os.environ.get("AWKWARD_HIP_LIBRARY") or os.path.join(HERE, "lib", "libawkward_hip.so")
 
 _lib = None
 
@@ -38,7 +40,8 @@ def lib() -> ctypes.CDLL:
                 f"{LIBPATH} is not built; run `python -m awkward_amd.build` "
                 "(the hip backend has no CPU fallback)"
             )
-        _warn_if_stale()
+        if "AWKWARD_HIP_LIBRARY" not in os.environ:
+            _warn_if_stale()
         handle = ctypes.CDLL(LIBPATH)
         handle.awkward_hip_set_stream.argtypes = [ctypes.c_void_p]
         handle.awkward_hip_set_stream.restype = None

@@ -121,6 +121,9 @@ def run_config1(ak, backend, torch, reps=5):
     read.restype = KernelError
     sink = torch.zeros(1, dtype=torch.int64, device="cuda")
     ops["read_ceiling"] = (lambda: read(dev_x.data_ptr(), 8 * N, sink.data_ptr()), None, 8 * N)
+    dispatch.reduce_sum(out_f, dev_x, dev_off, n, v)
+    torch.cuda.synchronize()
+    ACCURACY["config1"] = (off, x, out_f.cpu().numpy())
     for name, (key, args, nbytes) in list(ops.items()) + [("sum_best_dirty_scrub", ops["sum_best"])]:
         if args is None:
             call = key
@@ -422,6 +425,65 @@ def cpu_baseline(ak, off, fields, sample_events):
                       f"through the same host layer on the single-threaded C++ oracle"}
 
 
+ACCURACY = {}  # host copies of inputs and GPU sums for the fsum report
+
+
+def _fsum_errors(off, values, sums, exact):
+    """Max and mean |sum - fsum| / sum(|x|) over the lists (0 for empty lists)."""
+    import numpy as np
+
+    vals = np.concatenate([np.abs(values.astype(np.float64)), [0.0]])  # reduceat needs start < len
+    mags = np.add.reduceat(vals, off[:-1])
+    mags[off[1:] == off[:-1]] = 0.0
+    err = np.abs(sums.astype(np.float64) - exact)
+    rel = np.where(mags > 0, err / np.where(mags > 0, mags, 1.0), 0.0)
+    return {"max_rel_err": float(rel.max()) if len(rel) else 0.0, "mean_rel_err": float(rel.mean()) if len(rel) else 0.0}
+
+
+def accuracy_report(ak, off, fields, gpu_sum, sample_events):
+    """Float sums against math.fsum, for the GPU and the single-threaded CPU
+    oracle: config 1's float64 sums (all 1M lists) and the event pipeline's
+    float32 sum(pt) over the kept particles (the first `sample_events`
+    events).  Error = |sum - fsum| / sum(|x|) per list."""
+    import math
+
+    import numpy as np
+
+    from oracle.backend import OracleBackend
+
+    be = OracleBackend()
+    report = {"definition": "per list |sum - math.fsum| / sum(|x|); max and mean over lists"}
+    if "config1" in ACCURACY:
+        off1, x1, g1 = ACCURACY["config1"]
+        exact = np.array([math.fsum(x1[a:b]) for a, b in zip(off1[:-1], off1[1:])])
+        c1 = ak.sum(ak.from_offsets(off1, x1, be), axis=-1).layout
+        cpu1 = np.asarray(c1.nplike.to_numpy(c1.data))
+        report["config1_sum_f64"] = {"lists": len(exact), "gpu": _fsum_errors(off1, x1, g1, exact),
+                                     "cpu": _fsum_errors(off1, x1, cpu1, exact),
+                                     "gpu_bits_equal_cpu": float(np.mean(g1.view(np.int64) == cpu1.view(np.int64)))}
+    n = min(sample_events, len(off) - 1)
+    pt = fields["pt"][: int(off[n])]
+    eta = fields["eta"][: int(off[n])]
+    keep = (pt > np.float32(25)) & (np.abs(eta) < np.float32(2.4))
+    rank = np.cumsum(keep) - 1  # selected particles before each one, over all events
+    sel_before = np.concatenate([[0], np.cumsum(keep)])[off[:n]]
+    event = np.repeat(np.arange(n), np.diff(off[: n + 1]))
+    kept = keep & ((rank - sel_before[event]) >= 1)  # [:, 1:] of the selected particles
+    kept_off = np.concatenate([[0], np.cumsum(np.bincount(event[kept], minlength=n))])
+    kept_pt = pt[kept]
+    # sums of at most 20 float32 values of 5..~500 are exact in float64
+    exact2 = np.zeros(n)
+    np.add.at(exact2, event[kept], kept_pt.astype(np.float64))
+    sub = ak.from_offsets(off[: n + 1].copy(), {k: np.ascontiguousarray(v[: int(off[n])]) for k, v in fields.items()}, be)
+    c2 = pipeline(ak, sub)[1].layout
+    cpu2 = np.asarray(c2.nplike.to_numpy(c2.data))
+    g2 = gpu_sum[:n]
+    report["config2_sum_pt_f32"] = {"events": n, "gpu": _fsum_errors(kept_off, kept_pt, g2, exact2),
+                                    "cpu": _fsum_errors(kept_off, kept_pt, cpu2, exact2),
+                                    "gpu_bits_equal_cpu": float(np.mean(g2.view(np.int32) == cpu2.view(np.int32)))}
+    return report
+
+
 def time_form(torch, fn, steps, warmup, events):
     for _ in range(warmup):
         fn()
@@ -509,6 +571,7 @@ def main():
                             "what": "awkward_hip_event_pipeline, one kernel, replayed from a hipGraph"}
     log(f"fused_graph: {forms['fused_graph']}")
 
+    gpu_sum_pt = pipe.out["sum"][: args.cpu_sample].cpu().numpy()
     alg_bytes = pipe.algorithmic_bytes(N)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -531,6 +594,9 @@ def main():
         log(f"config5: learned regret {config5['learned']['aggregate']}, table {config5['table']['aggregate']}, "
             f"heuristic {config5['heuristic']['aggregate']} over {config5['rows']} held-out rows")
     cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
+    accuracy = None if args.no_cpu else accuracy_report(ak, off, fields, gpu_sum_pt, args.cpu_sample)
+    if accuracy:
+        log(f"accuracy: {json.dumps(accuracy)}")
 
     line = {
         "metric": METRIC,
@@ -552,6 +618,7 @@ def main():
         "speedup_fused_vs_plain": round(forms["fused_graph"]["events_per_s"] / forms["plain"]["events_per_s"], 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "config1": config1,
         "config3": config3,
         "config4": config4,
