@@ -57,6 +57,19 @@ FAMILIES = [
     "awkward_ListOffsetArray_local_preparenext_64",
     "awkward_sort",
     "awkward_argsort",
+    # option-type contents under the reducers and getitem
+    "awkward_ByteMaskedArray_numnull",
+    "awkward_ByteMaskedArray_reduce_next_64",
+    "awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64",
+    "awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64",
+    "awkward_ByteMaskedArray_getitem_nextcarry",
+    "awkward_ByteMaskedArray_getitem_nextcarry_outindex",
+    "awkward_IndexedArray_numnull",
+    "awkward_IndexedArray_reduce_next_64",
+    "awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64",
+    "awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64",
+    "awkward_IndexedArray_getitem_nextcarry_outindex",
+    "awkward_NumpyArray_reduce_adjust_starts_shifts_64",
 ]
 
 _DTYPE = {

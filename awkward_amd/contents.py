@@ -73,8 +73,8 @@ class NumpyArray(Content):
     def project(self):
         return self
 
-    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims):
-        out, outmask = reducer.apply(self, offsets, starts, None, outlength, mask)
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
+        out, outmask = reducer.apply(self, offsets, starts, shifts, outlength, mask)
         if mask and outmask is None:
             outmask = self.nplike.empty(outlength, np.int8)
             self._check(
@@ -179,8 +179,8 @@ class IndexedArray(Content):
     def project(self):
         return self.content._carry(self.index, False)
 
-    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims):
-        return self.project()._reduce_next(reducer, starts, offsets, outlength, mask, keepdims)
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
+        return self.project()._reduce_next(reducer, starts, offsets, outlength, mask, keepdims, shifts)
 
     def _argsort_next(self, offsets, ascending, stable):
         return self.project()._argsort_next(offsets, ascending, stable)
@@ -200,6 +200,35 @@ class IndexedArray(Content):
         return self.project().to_list()
 
 
+def _option_reduce_next(layout, reducer, starts, offsets, outlength, mask, keepdims, shifts, numnull_key,
+                        reduce_key, shift_keys, args):
+    """The local (axis=-1) branch of ByteMaskedArray._reduce_next
+    (contents/bytemaskedarray.py:848-975) and IndexedOptionArray._reduce_next
+    (contents/indexedoptionarray.py:1394-1431 with _rearrange_prepare_next
+    :1177-1227 and _rearrange_nextshifts :1136-1175): count the nulls, compact
+    the valid elements of each bin into the next carry and offsets, and for
+    positional reducers pass the nulls before each valid element down as
+    shifts, so argmin/argmax count them in their local index."""
+    nplike = layout.nplike
+    n = layout.length
+    numnull = nplike.empty(1, np.int64)
+    layout._check(layout._kernel(*numnull_key)(numnull, *args["numnull"]))
+    next_length = n - nplike.item(numnull, 0)
+    nextcarry = nplike.empty(next_length, np.int64)
+    nextoffsets = nplike.empty(outlength + 1, np.int64)
+    outindex = nplike.empty(n, np.int64)
+    layout._check(layout._kernel(*reduce_key)(nextcarry, nextoffsets, outindex, *args["reduce"](offsets, outlength)))
+    nextshifts = None
+    if reducer.needs_position:
+        nextshifts = nplike.empty(next_length, np.int64)
+        if shifts is None:
+            layout._check(layout._kernel(*shift_keys[0])(nextshifts, *args["shifts"]))
+        else:
+            layout._check(layout._kernel(*shift_keys[1])(nextshifts, *args["shifts"], shifts))
+    nxt = layout.content._carry(nextcarry, False)
+    return nxt._reduce_next(reducer, starts, nextoffsets, outlength, mask, keepdims, nextshifts)
+
+
 class ByteMaskedArray(Content):
     def __init__(self, mask, content, valid_when, backend):
         self.mask = mask
@@ -214,11 +243,107 @@ class ByteMaskedArray(Content):
     def __repr__(self):
         return f"ByteMaskedArray(valid_when={self.valid_when}, {self.content!r})"
 
+    def slice(self, start, stop):
+        return ByteMaskedArray(self.mask[start:stop], self.content.slice(start, stop), self.valid_when, self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        return ByteMaskedArray(self.nplike.take(self.mask, carry), self.content._carry(carry, allow_lazy),
+                               self.valid_when, self.backend)
+
+    def _project_valid(self):
+        """(carry of the valid elements, outindex with -1 at the nulls):
+        awkward_ByteMaskedArray_getitem_nextcarry_outindex_64."""
+        nplike = self.nplike
+        m = self._mask8()
+        n = self.length
+        numnull = nplike.empty(1, np.int64)
+        self._check(self._kernel("awkward_ByteMaskedArray_numnull", np.int64, np.int8)(numnull, m, n,
+                                                                                      bool(self.valid_when)))
+        nextcarry = nplike.empty(n - nplike.item(numnull, 0), np.int64)
+        outindex = nplike.empty(n, np.int64)
+        self._check(self._kernel("awkward_ByteMaskedArray_getitem_nextcarry_outindex", np.int64, np.int64, np.int8)(
+            nextcarry, outindex, m, n, bool(self.valid_when)))
+        return nextcarry, outindex
+
+    def _mask8(self):
+        m = self.mask
+        return self.nplike.view(m, np.int8) if self.nplike.dtype(m) == np.dtype(np.bool_) else m
+
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
+        m = self._mask8()
+        n = self.length
+        vw = bool(self.valid_when)
+        return _option_reduce_next(
+            self, reducer, starts, offsets, outlength, mask, keepdims, shifts,
+            ("awkward_ByteMaskedArray_numnull", np.int64, np.int8),
+            ("awkward_ByteMaskedArray_reduce_next_64", np.int64, np.int64, np.int64, np.int8, np.int64),
+            (("awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64", np.int64, np.int8),
+             ("awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64", np.int64, np.int8, np.int64)),
+            {"numnull": (m, n, vw), "reduce": lambda off, ol: (m, off, ol, vw), "shifts": (m, n, vw)},
+        )
+
     def to_list(self):
         mask = self.nplike.to_numpy(self.mask).astype(bool)
         values = self.content.to_list()
         valid = mask if self.valid_when else ~mask
         return [v if ok else None for v, ok in zip(values, valid)]
+
+
+class IndexedOptionArray(Content):
+    """Option type by index: element i is content[index[i]], or None where
+    index[i] < 0 (contents/indexedoptionarray.py)."""
+
+    def __init__(self, index, content, backend):
+        self.index = index
+        self.content = content
+        self.backend = backend
+
+    @property
+    def length(self):
+        return self.nplike.length(self.index)
+
+    def __repr__(self):
+        return f"IndexedOptionArray(len={self.length}, {self.content!r})"
+
+    def slice(self, start, stop):
+        return IndexedOptionArray(self.index[start:stop], self.content, self.backend)
+
+    def _carry(self, carry, allow_lazy=True):
+        return IndexedOptionArray(self.nplike.take(self.index, carry), self.content, self.backend)
+
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
+        ix = self.index
+        it = self.nplike.dtype(ix)
+        n = self.length
+        return _option_reduce_next(
+            self, reducer, starts, offsets, outlength, mask, keepdims, shifts,
+            ("awkward_IndexedArray_numnull", np.int64, it),
+            ("awkward_IndexedArray_reduce_next_64", np.int64, np.int64, np.int64, it, np.int64),
+            (("awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64", np.int64, it),
+             ("awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64", np.int64, it, np.int64)),
+            {"numnull": (ix, n), "reduce": lambda off, ol: (ix, off, ol), "shifts": (ix, n)},
+        )
+
+    def _project_valid(self):
+        """(carry of the valid elements, outindex with -1 at the nulls):
+        awkward_IndexedArray_getitem_nextcarry_outindex_64 (an index past the
+        content is an IndexError, as in the reference)."""
+        nplike = self.nplike
+        ix = self.index
+        it = nplike.dtype(ix)
+        n = self.length
+        numnull = nplike.empty(1, np.int64)
+        self._check(self._kernel("awkward_IndexedArray_numnull", np.int64, it)(numnull, ix, n))
+        nextcarry = nplike.empty(n - nplike.item(numnull, 0), np.int64)
+        outindex = nplike.empty(n, it)
+        self._check(self._kernel("awkward_IndexedArray_getitem_nextcarry_outindex", np.int64, it, it)(
+            nextcarry, outindex, ix, n, self.content.length))
+        return nextcarry, outindex
+
+    def to_list(self):
+        index = self.nplike.to_numpy(self.index).astype(np.int64)
+        values = self.content.to_list()
+        return [values[i] if i >= 0 else None for i in index]
 
 
 class RegularArray(Content):
@@ -492,7 +617,15 @@ _kSliceNone = 2**63 - 1
 
 def reduce(layout, reducer, mask, keepdims):
     """_do.reduce at axis=-1 (src/awkward/_do.py:225-340) for a list of
-    numbers: returns the reduced content (the outer length-1 level removed)."""
+    numbers (or of optional numbers): returns the reduced content (the outer
+    length-1 level removed).  Missing lists (an option type over the lists)
+    reduce their valid lists and stay missing in the result, as the
+    reference's option types do above the reduced axis
+    (contents/indexedoptionarray.py:1433-1470)."""
+    if isinstance(layout, (IndexedOptionArray, ByteMaskedArray)) and getattr(layout.content, "is_list", False):
+        nextcarry, outindex = layout._project_valid()
+        out = reduce(layout.content._carry(nextcarry, False), reducer, mask, keepdims)
+        return IndexedOptionArray(outindex, out, layout.backend)
     if isinstance(layout, ListArray):
         layout = layout.to_ListOffsetArray64()
     if not isinstance(layout, ListOffsetArray):
@@ -509,6 +642,7 @@ __all__ = [
     "RecordArray",
     "IndexedArray",
     "ByteMaskedArray",
+    "IndexedOptionArray",
     "RegularArray",
     "ListOffsetArray",
     "ListArray",

@@ -320,11 +320,28 @@ def from_offsets(offsets, content, backend):
 
 
 def from_list(lists, backend, dtype=None):
-    """A list of lists of numbers (or of dicts of numbers) as an Array."""
+    """A list of lists of numbers (or of dicts of numbers) as an Array.
+    None among the numbers makes an option type: IndexedOptionArray(int64
+    index, -1 for None) over the non-None values, the layout the reference's
+    ak.from_iter builds for optional numbers."""
+    if any(x is None for x in lists):  # missing lists: an option type over the lists
+        present = [x for x in lists if x is not None]
+        index = np.full(len(lists), -1, dtype=np.int64)
+        index[[i for i, x in enumerate(lists) if x is not None]] = np.arange(len(present))
+        inner = from_list(present, backend, dtype).layout
+        return Array(C.IndexedOptionArray(backend.nplike.asarray(index), inner, backend))
     counts = [len(x) for x in lists]
     offsets = np.zeros(len(lists) + 1, dtype=np.int64)
     np.cumsum(counts, out=offsets[1:])
     flat = [v for x in lists for v in x]
+    if any(v is None for v in flat):
+        present = [v for v in flat if v is not None]
+        index = np.full(len(flat), -1, dtype=np.int64)
+        index[[i for i, v in enumerate(flat) if v is not None]] = np.arange(len(present))
+        values = np.asarray(present, dtype=dtype if dtype is not None else (np.float64 if not present else None))
+        nplike = backend.nplike
+        inner = C.IndexedOptionArray(nplike.asarray(index), C.NumpyArray(nplike.asarray(values), backend), backend)
+        return Array(C.ListOffsetArray(nplike.asarray(offsets), inner, backend))
     if flat and isinstance(flat[0], dict):
         keys = list(flat[0])
         content = {k: np.asarray([r[k] for r in flat], dtype=dtype) for k in keys}
@@ -363,6 +380,8 @@ def _convert(layout, backend):
         return C.IndexedArray(move(layout.index), _convert(layout.content, backend), backend)
     if isinstance(layout, C.ByteMaskedArray):
         return C.ByteMaskedArray(move(layout.mask), _convert(layout.content, backend), layout.valid_when, backend)
+    if isinstance(layout, C.IndexedOptionArray):
+        return C.IndexedOptionArray(move(layout.index), _convert(layout.content, backend), backend)
     raise NotImplementedError(type(layout).__name__)
 
 

@@ -236,6 +236,10 @@ def to_buffers(
         elif isinstance(node, C.RecordArray):
             form = {"class": "RecordArray", "fields": None if node.fields is None else list(node.fields)}
             form["contents"] = [walk(c) for c in node.contents]
+        elif isinstance(node, C.IndexedOptionArray):
+            form = {"class": "IndexedOptionArray", "index": _index_name(node.index)}
+            put(getkey(key, "index"), node.index)
+            form["content"] = walk(node.content)
         elif isinstance(node, C.IndexedArray):
             form = {"class": "IndexedArray", "index": _index_name(node.index)}
             put(getkey(key, "index"), node.index)
@@ -333,6 +337,10 @@ def _reconstitute(form, length, container, getkey, backend):
         index = _to_int64(buffer("index", _INDEX_DTYPES[form["index"]], length))
         inner = 0 if length == 0 else builtins.max(0, int(index.max()) + 1)
         return C.IndexedArray(index, _reconstitute(form["content"], inner, container, getkey, backend), backend)
+    if cls == "IndexedOptionArray":
+        index = buffer("index", _INDEX_DTYPES[form["index"]], length)
+        inner = 0 if length == 0 else builtins.max(0, int(index.max()) + 1)
+        return C.IndexedOptionArray(index, _reconstitute(form["content"], inner, container, getkey, backend), backend)
     if cls == "ByteMaskedArray":
         mask = buffer("mask", _INDEX_DTYPES[form["mask"]], length)
         content = _reconstitute(form["content"], length, container, getkey, backend)

@@ -135,7 +135,7 @@ class _Arg(Reducer):
             data = nplike.view(data, np.int8)
             dtype = np.dtype(np.int8)
         result = nplike.empty(outlength, np.int64)
-        if _fused(array):
+        if _fused(array) and shifts is None:
             from awkward_amd import dispatch
 
             outmask = nplike.empty(outlength, np.int8) if mask else None
@@ -145,12 +145,19 @@ class _Arg(Reducer):
         array._check(
             array.backend[self.kernel, np.int64, dtype, np.int64, np.int64](result, data, offsets, starts, outlength)
         )
-        # apply_positional_corrections (shifts is None on this path)
-        array._check(
-            array.backend["awkward_NumpyArray_reduce_adjust_starts_64", np.int64, np.int64, np.int64](
-                result, outlength, offsets, starts
+        # apply_positional_corrections: below an option type the shifts count
+        # the nulls removed before each element
+        if shifts is None:
+            array._check(
+                array.backend["awkward_NumpyArray_reduce_adjust_starts_64", np.int64, np.int64, np.int64](
+                    result, outlength, offsets, starts
+                )
             )
-        )
+        else:
+            array._check(
+                array.backend["awkward_NumpyArray_reduce_adjust_starts_shifts_64", np.int64, np.int64, np.int64,
+                              np.int64](result, outlength, offsets, starts, shifts)
+            )
         return type(array)(result, array.backend), None
 
 

@@ -424,6 +424,74 @@ struct Error awkward_argsort_float64_uint64(int64_t* toptr, const double* frompt
 /* replaces awkward-cpp/src/cpu-kernels/awkward_argsort.cpp:174; contract kernel-specification.yml:4111 */
 struct Error awkward_argsort_float64_int64(int64_t* toptr, const double* fromptr, int64_t length, const int64_t* offsets, int64_t offsetslength, bool ascending, bool stable);
 
+/* ---- awkward_ByteMaskedArray_numnull ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_numnull.cpp:3; contract kernel-specification.yml:196 */
+struct Error awkward_ByteMaskedArray_numnull(int64_t* numnull, const int8_t* mask, int64_t length, bool validwhen);
+
+/* ---- awkward_ByteMaskedArray_reduce_next_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_reduce_next_64.cpp:3; contract kernel-specification.yml:231 */
+struct Error awkward_ByteMaskedArray_reduce_next_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, const int8_t* mask, const int64_t* offsets, int64_t outlength, bool validwhen);
+
+/* ---- awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64.cpp:3; contract kernel-specification.yml:259 */
+struct Error awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64(int64_t* nextshifts, const int8_t* mask, int64_t length, bool valid_when);
+
+/* ---- awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64.cpp:3; contract kernel-specification.yml:282 */
+struct Error awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* nextshifts, const int8_t* mask, int64_t length, bool valid_when, const int64_t* shifts);
+
+/* ---- awkward_ByteMaskedArray_getitem_nextcarry ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_getitem_nextcarry.cpp:22; contract kernel-specification.yml:156 */
+struct Error awkward_ByteMaskedArray_getitem_nextcarry_64(int64_t* tocarry, const int8_t* mask, int64_t length, bool validwhen);
+
+/* ---- awkward_ByteMaskedArray_getitem_nextcarry_outindex ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ByteMaskedArray_getitem_nextcarry_outindex.cpp:27; contract kernel-specification.yml:174 */
+struct Error awkward_ByteMaskedArray_getitem_nextcarry_outindex_64(int64_t* tocarry, int64_t* outindex, const int8_t* mask, int64_t length, bool validwhen);
+
+/* ---- awkward_IndexedArray_numnull ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_numnull.cpp:26; contract kernel-specification.yml:573 */
+struct Error awkward_IndexedArray32_numnull(int64_t* numnull, const int32_t* fromindex, int64_t lenindex);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_numnull.cpp:28; contract kernel-specification.yml:578 */
+struct Error awkward_IndexedArray64_numnull(int64_t* numnull, const int64_t* fromindex, int64_t lenindex);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_numnull.cpp:27; contract kernel-specification.yml:583 */
+struct Error awkward_IndexedArrayU32_numnull(int64_t* numnull, const uint32_t* fromindex, int64_t lenindex);
+
+/* ---- awkward_IndexedArray_reduce_next_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_64.cpp:38; contract kernel-specification.yml:707 */
+struct Error awkward_IndexedArray32_reduce_next_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, const int32_t* index, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_64.cpp:40; contract kernel-specification.yml:715 */
+struct Error awkward_IndexedArray64_reduce_next_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, const int64_t* index, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_64.cpp:39; contract kernel-specification.yml:723 */
+struct Error awkward_IndexedArrayU32_reduce_next_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, const uint32_t* index, const int64_t* offsets, int64_t outlength);
+
+/* ---- awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64.cpp:31; contract kernel-specification.yml:799 */
+struct Error awkward_IndexedArray32_reduce_next_nonlocal_nextshifts_64(int64_t* nextshifts, const int32_t* index, int64_t length);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64.cpp:33; contract kernel-specification.yml:804 */
+struct Error awkward_IndexedArray64_reduce_next_nonlocal_nextshifts_64(int64_t* nextshifts, const int64_t* index, int64_t length);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64.cpp:32; contract kernel-specification.yml:809 */
+struct Error awkward_IndexedArrayU32_reduce_next_nonlocal_nextshifts_64(int64_t* nextshifts, const uint32_t* index, int64_t length);
+
+/* ---- awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64.cpp:32; contract kernel-specification.yml:829 */
+struct Error awkward_IndexedArray32_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* nextshifts, const int32_t* index, int64_t length, const int64_t* shifts);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64.cpp:34; contract kernel-specification.yml:835 */
+struct Error awkward_IndexedArray64_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* nextshifts, const int64_t* index, int64_t length, const int64_t* shifts);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64.cpp:33; contract kernel-specification.yml:841 */
+struct Error awkward_IndexedArrayU32_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* nextshifts, const uint32_t* index, int64_t length, const int64_t* shifts);
+
+/* ---- awkward_IndexedArray_getitem_nextcarry_outindex ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_getitem_nextcarry_outindex.cpp:37; contract kernel-specification.yml:507 */
+struct Error awkward_IndexedArray32_getitem_nextcarry_outindex_64(int64_t* tocarry, int32_t* toindex, const int32_t* fromindex, int64_t lenindex, int64_t lencontent);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_getitem_nextcarry_outindex.cpp:39; contract kernel-specification.yml:514 */
+struct Error awkward_IndexedArray64_getitem_nextcarry_outindex_64(int64_t* tocarry, int64_t* toindex, const int64_t* fromindex, int64_t lenindex, int64_t lencontent);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_IndexedArray_getitem_nextcarry_outindex.cpp:38; contract kernel-specification.yml:521 */
+struct Error awkward_IndexedArrayU32_getitem_nextcarry_outindex_64(int64_t* tocarry, uint32_t* toindex, const uint32_t* fromindex, int64_t lenindex, int64_t lencontent);
+
+/* ---- awkward_NumpyArray_reduce_adjust_starts_shifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_NumpyArray_reduce_adjust_starts_shifts_64.cpp:3; contract kernel-specification.yml:2566 */
+struct Error awkward_NumpyArray_reduce_adjust_starts_shifts_64(int64_t* toptr, int64_t outlength, const int64_t* offsets, const int64_t* starts, const int64_t* shifts);
+
 /* ---- runtime (no counterpart in awkward-cpp, which is synchronous CPU code) ---- */
 
 /* Stream that subsequent entry points enqueue on (a hipStream_t; NULL = the

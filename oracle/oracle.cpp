@@ -596,6 +596,186 @@ Error awkward_ListOffsetArray_local_preparenext_64(int64_t* tocarry, const int64
   return ok();
 }
 
+// ---- option-type contents --------------------------------------------------
+// awkward_ByteMaskedArray_numnull.cpp
+Error awkward_ByteMaskedArray_numnull(int64_t* numnull, const int8_t* mask, int64_t length, bool validwhen) {
+  *numnull = 0;
+  for (int64_t i = 0; i < length; i++)
+    if ((mask[i] != 0) != validwhen) *numnull = *numnull + 1;
+  return ok();
+}
+
+// awkward_ByteMaskedArray_reduce_next_64.cpp: the valid elements of each bin,
+// in bin order, become the next carry; outindex maps each element to its
+// rank among them (-1 if masked); nextoffsets counts them per bin.
+Error awkward_ByteMaskedArray_reduce_next_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex,
+                                             const int8_t* mask, const int64_t* offsets, int64_t outlength,
+                                             bool validwhen) {
+  int64_t k = 0;
+  nextoffsets[0] = 0;
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    for (int64_t i = offsets[bin]; i < offsets[bin + 1]; i++) {
+      if ((mask[i] != 0) == validwhen) {
+        nextcarry[k] = i;
+        outindex[i] = k++;
+      } else {
+        outindex[i] = -1;
+      }
+    }
+    nextoffsets[bin + 1] = k;
+  }
+  return ok();
+}
+
+// awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64.cpp and
+// ..._fromshifts_64.cpp: for each valid element, the masked ones before it
+// (plus its incoming shift).
+Error awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_64(int64_t* nextshifts, const int8_t* mask,
+                                                                 int64_t length, bool valid_when) {
+  int64_t nullsum = 0, k = 0;
+  for (int64_t i = 0; i < length; i++) {
+    if ((mask[i] != 0) == valid_when) nextshifts[k++] = nullsum;
+    else nullsum++;
+  }
+  return ok();
+}
+Error awkward_ByteMaskedArray_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* nextshifts, const int8_t* mask,
+                                                                           int64_t length, bool valid_when,
+                                                                           const int64_t* shifts) {
+  int64_t nullsum = 0, k = 0;
+  for (int64_t i = 0; i < length; i++) {
+    if ((mask[i] != 0) == valid_when) nextshifts[k++] = shifts[i] + nullsum;
+    else nullsum++;
+  }
+  return ok();
+}
+
+// awkward_ByteMaskedArray_getitem_nextcarry.cpp and ..._outindex.cpp
+Error awkward_ByteMaskedArray_getitem_nextcarry_64(int64_t* tocarry, const int8_t* mask, int64_t length,
+                                                   bool validwhen) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < length; i++)
+    if ((mask[i] != 0) == validwhen) tocarry[k++] = i;
+  return ok();
+}
+Error awkward_ByteMaskedArray_getitem_nextcarry_outindex_64(int64_t* tocarry, int64_t* outindex, const int8_t* mask,
+                                                            int64_t length, bool validwhen) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < length; i++) {
+    if ((mask[i] != 0) == validwhen) {
+      tocarry[k] = i;
+      outindex[i] = k++;
+    } else {
+      outindex[i] = -1;
+    }
+  }
+  return ok();
+}
+
+// awkward_NumpyArray_reduce_adjust_starts_shifts_64.cpp: a global arg index
+// into the compacted content becomes local to the bin, counting the nulls
+// that were removed before it.
+Error awkward_NumpyArray_reduce_adjust_starts_shifts_64(int64_t* toptr, int64_t outlength, const int64_t*,
+                                                        const int64_t* starts, const int64_t* shifts) {
+  for (int64_t k = 0; k < outlength; k++) {
+    const int64_t i = toptr[k];
+    if (i >= 0) toptr[k] += shifts[i] - starts[k];
+  }
+  return ok();
+}
+
+}  // extern "C"
+
+namespace {
+
+// awkward_IndexedArray_numnull.cpp (negative index = null; an unsigned
+// index is never null)
+template <typename C>
+Error indexed_numnull(int64_t* numnull, const C* fromindex, int64_t lenindex) {
+  *numnull = 0;
+  for (int64_t i = 0; i < lenindex; i++)
+    if (fromindex[i] < 0) *numnull = *numnull + 1;
+  return ok();
+}
+
+// awkward_IndexedArray_reduce_next_64.cpp
+template <typename C>
+Error indexed_reduce_next(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, const C* index,
+                          const int64_t* offsets, int64_t outlength) {
+  int64_t k = 0;
+  nextoffsets[0] = 0;
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    for (int64_t i = offsets[bin]; i < offsets[bin + 1]; i++) {
+      if (index[i] >= 0) {
+        nextcarry[k] = (int64_t)index[i];
+        outindex[i] = k++;
+      } else {
+        outindex[i] = -1;
+      }
+    }
+    nextoffsets[bin + 1] = k;
+  }
+  return ok();
+}
+
+// awkward_IndexedArray_reduce_next_nonlocal_nextshifts_64.cpp and ..._fromshifts_64.cpp
+template <typename C>
+Error indexed_nextshifts(int64_t* nextshifts, const C* index, int64_t length, const int64_t* shifts) {
+  int64_t nullsum = 0, k = 0;
+  for (int64_t i = 0; i < length; i++) {
+    if (index[i] >= 0) nextshifts[k++] = (shifts ? shifts[i] : 0) + nullsum;
+    else nullsum++;
+  }
+  return ok();
+}
+
+// awkward_IndexedArray_getitem_nextcarry_outindex.cpp:13-28: an index at or
+// past lencontent is an error (identity i, attempt j); a negative one is null.
+template <typename C>
+Error indexed_nextcarry_outindex(int64_t* tocarry, C* toindex, const C* fromindex, int64_t lenindex,
+                                 int64_t lencontent) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < lenindex; i++) {
+    const C j = fromindex[i];
+    if ((int64_t)j >= lencontent) {
+      return fail("index out of range", i, (int64_t)j);
+    } else if (j < 0) {
+      toindex[i] = (C)-1;
+    } else {
+      tocarry[k] = (int64_t)j;
+      toindex[i] = (C)k;
+      k++;
+    }
+  }
+  return ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+#define OPTION_INDEXED(P, C)                                                                                     \
+  Error awkward_IndexedArray##P##_numnull(int64_t* n, const C* fi, int64_t li) { return indexed_numnull<C>(n, fi, li); } \
+  Error awkward_IndexedArray##P##_reduce_next_64(int64_t* nc, int64_t* no, int64_t* oi, const C* ix,             \
+                                                 const int64_t* off, int64_t ol) {                              \
+    return indexed_reduce_next<C>(nc, no, oi, ix, off, ol);                                                     \
+  }                                                                                                            \
+  Error awkward_IndexedArray##P##_reduce_next_nonlocal_nextshifts_64(int64_t* ns, const C* ix, int64_t len) {    \
+    return indexed_nextshifts<C>(ns, ix, len, nullptr);                                                          \
+  }                                                                                                            \
+  Error awkward_IndexedArray##P##_reduce_next_nonlocal_nextshifts_fromshifts_64(int64_t* ns, const C* ix,        \
+                                                                               int64_t len, const int64_t* sh) { \
+    return indexed_nextshifts<C>(ns, ix, len, sh);                                                               \
+  }                                                                                                            \
+  Error awkward_IndexedArray##P##_getitem_nextcarry_outindex_64(int64_t* tc, C* ti, const C* fi, int64_t li,     \
+                                                                int64_t lc) {                                   \
+    return indexed_nextcarry_outindex<C>(tc, ti, fi, li, lc);                                                   \
+  }
+OPTION_INDEXED(32, int32_t)
+OPTION_INDEXED(U32, uint32_t)
+OPTION_INDEXED(64, int64_t)
+#undef OPTION_INDEXED
+
 #define SORT(TNAME, T)                                                                          \
   Error awkward_sort_##TNAME(T* t, const T* f, int64_t len, const int64_t* o, int64_t ol, bool a, \
                              bool s) {                                                          \
