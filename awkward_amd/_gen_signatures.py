@@ -70,6 +70,11 @@ FAMILIES = [
     "awkward_IndexedArray_reduce_next_nonlocal_nextshifts_fromshifts_64",
     "awkward_IndexedArray_getitem_nextcarry_outindex",
     "awkward_NumpyArray_reduce_adjust_starts_shifts_64",
+    # non-local (axis != -1) reductions
+    "awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64",
+    "awkward_ListOffsetArray_reduce_nonlocal_preparenext_64",
+    "awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64",
+    "awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64",
 ]
 
 _DTYPE = {

@@ -23,6 +23,8 @@ from __future__ import annotations
 
 import numpy as np
 
+builtins_min = min
+
 from awkward_amd import reducers as _reducers
 
 
@@ -46,12 +48,30 @@ class Content:
     def is_list(self):
         return False
 
+    @property
+    def depth(self):
+        """List nesting depth counted from the numbers (NumpyArray: 1),
+        Content.branch_depth's depth for a non-branching layout."""
+        return self.content.depth
+
+    def _reduce_axis(self, reducer, negaxis, starts, shifts, offsets, outlength, mask, keepdims):
+        """Content._reduce_next of the reference for any negaxis: the
+        numbers' level is the local reduction (negaxis 1)."""
+        if negaxis != 1:
+            raise NotImplementedError(f"a reduction {negaxis} levels above the numbers through a "
+                                      f"{type(self).__name__}")
+        return self._reduce_next(reducer, starts, offsets, outlength, mask, keepdims, shifts)
+
 
 # ---- leaves ---------------------------------------------------------------------
 class NumpyArray(Content):
     def __init__(self, data, backend):
         self.data = data
         self.backend = backend
+
+    @property
+    def depth(self):
+        return 1
 
     @property
     def length(self):
@@ -111,6 +131,10 @@ class NumpyArray(Content):
 
 
 class RecordArray(Content):
+    @property
+    def depth(self):
+        raise NotImplementedError("reductions of records are not on the hip path (reduce a field)")
+
     def __init__(self, contents, fields, length, backend):
         self.contents = list(contents)
         self.fields = None if fields is None else list(fields)
@@ -354,6 +378,26 @@ class RegularArray(Content):
         self.backend = backend
 
     @property
+    def depth(self):
+        return 1 + self.content.depth
+
+    @property
+    def is_list(self):
+        return True
+
+    def slice(self, start, stop):
+        return RegularArray(self.content.slice(start * self.size, stop * self.size), self.size, stop - start,
+                            self.backend)
+
+    def to_ListOffsetArray64(self):
+        offsets = self.nplike.arange(self.length + 1, np.int64) * self.size
+        return ListOffsetArray(offsets, self.content.slice(0, self.length * self.size), self.backend)
+
+    def _reduce_axis(self, reducer, negaxis, starts, shifts, offsets, outlength, mask, keepdims):
+        return self.to_ListOffsetArray64()._reduce_axis(reducer, negaxis, starts, shifts, offsets, outlength, mask,
+                                                        keepdims)
+
+    @property
     def length(self):
         return self._length
 
@@ -411,6 +455,74 @@ class ListOffsetArray(Content):
         if first != 0:
             offsets = offsets - first
         return offsets, self.content.slice(first, last)
+
+    @property
+    def depth(self):
+        return 1 + self.content.depth
+
+    def to_ListOffsetArray64(self):
+        return self
+
+    def slice(self, start, stop):
+        return ListOffsetArray(self.offsets[start:stop + 1], self.content, self.backend)
+
+    # -- reduce at any axis: listoffsetarray.py:1263-1431 --
+    def _reduce_axis(self, reducer, negaxis, starts, shifts, offsets, outlength, mask, keepdims):
+        nplike = self.nplike
+        selfoffsets, trimmed = self._zero_based()
+        depth = self.depth
+        if negaxis == depth:
+            # non-local: this level's lists are regrouped by column
+            (distincts, maxcount, maxnextparents, nextcarry, nextoffsets,
+             nextstarts) = self._rearrange_prepare_next(selfoffsets, trimmed, outlength, offsets)
+            outstarts = nplike.empty(outlength, np.int64)
+            outstops = nplike.empty(outlength, np.int64)
+            self._check(self._kernel("awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64", np.int64, np.int64,
+                                     np.int64)(outstarts, outstops, distincts, nplike.length(distincts), outlength))
+            nextshifts = None
+            if reducer.needs_position:
+                nextlen = nplike.length(nextcarry)
+                nextshifts = nplike.empty(nextlen, np.int64)
+                nummissing = nplike.empty(maxcount, np.int64)
+                missing = nplike.empty(nplike.item(selfoffsets, self.length), np.int64)
+                self._check(self._kernel("awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64", np.int64, np.int64,
+                                         np.int64, np.int64, np.int64, np.int64, np.int64)(
+                    nummissing, missing, nextshifts, selfoffsets, self.length, starts, offsets, outlength, maxcount,
+                    nextlen, nextcarry))
+            nextcontent = trimmed._carry(nextcarry, False)
+            outcontent = nextcontent._reduce_axis(reducer, negaxis - 1, nextstarts, nextshifts, nextoffsets,
+                                                  maxnextparents + 1, mask, False)
+            out = ListArray(outstarts, outstops, outcontent, self.backend)
+            if keepdims:
+                out = RegularArray(out, 1, self.length, self.backend)
+            return out
+        # local: the reduction is below this level
+        outcontent = trimmed._reduce_axis(reducer, negaxis, selfoffsets[:-1], shifts, selfoffsets, self.length, mask,
+                                          keepdims)
+        if not (keepdims and depth == negaxis + 1) and depth >= negaxis + 2:
+            outcontent = outcontent.to_ListOffsetArray64()
+        return ListOffsetArray(offsets, outcontent, self.backend)
+
+    def _rearrange_prepare_next(self, selfoffsets, trimmed, outlength, offsets):
+        """listoffsetarray.py:1432-1526 on the zero-based lists."""
+        nplike = self.nplike
+        nextlen = builtins_min(nplike.item(selfoffsets, self.length), trimmed.length)
+        maxcount_ = nplike.empty(1, np.int64)
+        offsetscopy = nplike.empty(self.length + 1, np.int64)
+        self._check(self._kernel("awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64", np.int64, np.int64,
+                                 np.int64)(maxcount_, offsetscopy, selfoffsets, self.length))
+        maxcount = nplike.item(maxcount_, 0)
+        nextcarry = nplike.empty(nextlen, np.int64)
+        maxnextparents_ = nplike.empty(1, np.int64)
+        distincts = nplike.empty(outlength * maxcount, np.int64)
+        nextoffsets = nplike.empty(outlength * maxcount + 1, np.int64)
+        self._check(self._kernel("awkward_ListOffsetArray_reduce_nonlocal_preparenext_64", np.int64, np.int64,
+                                 np.int64, np.int64, np.int64, np.int64, np.int64)(
+            nextcarry, nextoffsets, nextlen, maxnextparents_, distincts, outlength * maxcount, offsetscopy,
+            selfoffsets, self.length, offsets, outlength, maxcount))
+        maxnextparents = nplike.item(maxnextparents_, 0)
+        nextoffsets = nextoffsets[: maxnextparents + 2]
+        return distincts, maxcount, maxnextparents, nextcarry, nextoffsets, nextoffsets[:-1]
 
     # -- reduce, axis=-1: listoffsetarray.py:1394-1430 --
     def _reduce_local(self, reducer, mask, keepdims):
@@ -575,6 +687,17 @@ class ListArray(Content):
         )
         return self._getitem_next_jagged(multistarts, multistops, item.content.data)
 
+    @property
+    def depth(self):
+        return 1 + self.content.depth
+
+    def _reduce_axis(self, reducer, negaxis, starts, shifts, offsets, outlength, mask, keepdims):
+        return self.to_ListOffsetArray64()._reduce_axis(reducer, negaxis, starts, shifts, offsets, outlength, mask,
+                                                        keepdims)
+
+    def slice(self, start, stop):
+        return ListArray(self.starts[start:stop], self.stops[start:stop], self.content, self.backend)
+
     def to_ListOffsetArray64(self):
         nplike = self.nplike
         starts, stops = self._contig()
@@ -615,17 +738,43 @@ def _ranges_carry(listarray, offsets):
 _kSliceNone = 2**63 - 1
 
 
-def reduce(layout, reducer, mask, keepdims):
-    """_do.reduce at axis=-1 (src/awkward/_do.py:225-340) for a list of
-    numbers (or of optional numbers): returns the reduced content (the outer
-    length-1 level removed).  Missing lists (an option type over the lists)
-    reduce their valid lists and stay missing in the result, as the
-    reference's option types do above the reduced axis
+def _first(out):
+    """Element 0 of a length-1 list level (_do.reduce's ``next[0]``)."""
+    nplike = out.nplike
+    if isinstance(out, RegularArray):
+        return out.content.slice(0, out.size)
+    if isinstance(out, ListOffsetArray):
+        return out.content.slice(nplike.item(out.offsets, 0), nplike.item(out.offsets, 1))
+    if isinstance(out, ListArray):
+        return out.content.slice(nplike.item(out.starts, 0), nplike.item(out.stops, 0))
+    raise NotImplementedError(type(out).__name__)
+
+
+def reduce(layout, reducer, mask, keepdims, axis=-1):
+    """_do.reduce (src/awkward/_do.py:225-340): returns the reduced layout
+    (the outer length-1 level removed).  axis=-1 of a list of numbers (or of
+    optional numbers) is the local path; higher axes regroup the lists by
+    column first (the non-local branch).  Missing lists (an option type over
+    the lists) reduce their valid lists and stay missing in the result, as
+    the reference's option types do above the reduced axis
     (contents/indexedoptionarray.py:1433-1470)."""
+    depth = layout.depth
+    negaxis = -axis if axis < 0 else depth - axis
+    if not 0 < negaxis <= depth:
+        raise ValueError(f"axis={axis} exceeds the depth of the nested list structure (which is {depth})")
     if isinstance(layout, (IndexedOptionArray, ByteMaskedArray)) and getattr(layout.content, "is_list", False):
         nextcarry, outindex = layout._project_valid()
-        out = reduce(layout.content._carry(nextcarry, False), reducer, mask, keepdims)
+        out = reduce(layout.content._carry(nextcarry, False), reducer, mask, keepdims, axis)
+        if negaxis == depth:  # the missing lists are the ones reduced over: they contribute nothing
+            return out
         return IndexedOptionArray(outindex, out, layout.backend)
+    if depth == 1:
+        raise NotImplementedError("the hip path reduces list arrays (a flat array's full reduction is not on it)")
+    if negaxis != 1 or depth > 2:
+        nplike = layout.nplike
+        starts = nplike.asarray(np.zeros(1, dtype=np.int64))
+        offsets = nplike.asarray(np.array([0, layout.length], dtype=np.int64))
+        return _first(layout._reduce_axis(reducer, negaxis, starts, None, offsets, 1, mask, keepdims))
     if isinstance(layout, ListArray):
         layout = layout.to_ListOffsetArray64()
     if not isinstance(layout, ListOffsetArray):

@@ -1,0 +1,361 @@
+// nonlocal.hip -- reductions at an axis above the innermost lists
+// (axis != -1): the column regrouping of ListOffsetArray._reduce_next's
+// non-local branch.
+//
+// Replaces awkward-cpp/src/cpu-kernels/
+//   awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64.cpp
+//   awkward_ListOffsetArray_reduce_nonlocal_preparenext_64.cpp
+//   awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64.cpp
+//   awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64.cpp
+//
+// The CPU loops walk, per outer bin, every column j < maxcount and every row
+// of the bin, emitting the element in column j of each row long enough:
+// the bin's elements regrouped column-major, rows in order within a column.
+// On the device that order is a stable segmented sort of the elements by
+// column, one segment per outer bin (the library's own argsort), and a
+// histogram of (bin, column) groups scanned into their offsets.  An element
+// at sorted position t of bin b then lands at nextcarry index
+// t - segment start + (offset of group (b, 0)); its rank r within its
+// column group gives the rows of the bin before it that lack that column,
+// (row - first row of the bin) - r, which is nextshifts' `missing`.
+
+#include "errors.h"
+#include "scan.h"
+
+extern "C" Error awkward_argsort_int64_int64(int64_t* toptr, const int64_t* fromptr, int64_t length,
+                                             const int64_t* offsets, int64_t offsetslength, bool ascending,
+                                             bool stable);
+
+namespace akhip {
+namespace {
+
+const char* kFile = "awkward_amd/csrc/nonlocal.hip";
+constexpr int kBlock = 256;
+
+inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// first index in [lo, hi) with a[idx] > v (hi if none)
+__device__ inline int64_t upper_bound(const int64_t* a, int64_t lo, int64_t hi, int64_t v) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void set_i64(int64_t* p, int64_t v) { *p = v; }
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(int64_t* p, int64_t n, int64_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// ---- maxcount_offsetscopy ---------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void maxcount_kernel(unsigned long long* maxcount, int64_t* offsetscopy,
+                                                          const int64_t* offsets, int64_t length) {
+  __shared__ unsigned long long s_max[kBlock / kWave];
+  unsigned long long m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < length; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t c = offsets[i + 1] - offsets[i];
+    if (c > 0 && (unsigned long long)c > m) m = (unsigned long long)c;
+    offsetscopy[i + 1] = offsets[i + 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) offsetscopy[0] = offsets[0];
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) s_max[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; w++) m = s_max[w] > m ? s_max[w] : m;
+    if (m > 0) atomicMax(maxcount, m);
+  }
+}
+
+// ---- outstartsstops -----------------------------------------------------------------
+// group k covers distincts[k * maxcount, (k + 1) * maxcount); its stop is one
+// past the last entry that is not -1 (or its start)
+__global__ __launch_bounds__(kBlock) void outstartsstops_kernel(int64_t* outstarts, int64_t* outstops,
+                                                                const int64_t* distincts, int64_t lendistincts,
+                                                                int64_t outlength, int64_t maxcount) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= outlength) return;
+  const int64_t a = k * maxcount;
+  if (a >= lendistincts) return;
+  const int64_t b = a + maxcount < lendistincts ? a + maxcount : lendistincts;
+  int64_t stop = a;
+  for (int64_t i = b - 1; i >= a; i--)
+    if (distincts[i] != -1) {
+      stop = i + 1;
+      break;
+    }
+  outstarts[k] = a;
+  outstops[k] = stop;
+}
+
+// the CPU loop itself, for the degenerate split lendistincts < outlength
+__global__ void outstartsstops_serial(int64_t* outstarts, int64_t* outstops, const int64_t* distincts,
+                                      int64_t lendistincts, int64_t outlength) {
+  const int64_t maxcount = lendistincts / outlength;
+  int64_t k = 0, i_next = 0;
+  for (int64_t i = 0; i < lendistincts; i++) {
+    if (i == i_next) {
+      i_next += maxcount;
+      if (k < outlength) {
+        outstarts[k] = i;
+        outstops[k] = i;
+      }
+      k++;
+    }
+    if (distincts[i] != -1 && k - 1 < outlength) outstops[k - 1] = i + 1;
+  }
+}
+
+// ---- the column-major regrouping ------------------------------------------------------
+struct ColMajor {
+  int64_t row0, row1;  // rows of the outer bins: [outer_offsets[0], outer_offsets[outlength])
+  int64_t E0, n;       // their elements: [offsets[row0], offsets[row1])
+  int64_t groups;      // outlength * maxcount
+  int64_t* keys;       // n: column of each element (maxcount: past the last column)
+  int64_t* segoff;     // outlength + 1: each bin's first element, relative to E0
+  int64_t* order;      // n: argsort of keys within each bin (local positions)
+  int64_t* cnt;        // groups: elements per (bin, column)
+  int64_t* prefix;     // groups + 1: prefix[g] = elements of the groups before g
+  unsigned long long* maxgroup;  // 1 + the last non-empty group (0: none)
+};
+
+__global__ __launch_bounds__(kBlock) void keys_kernel(ColMajor cm, const int64_t* offsets,
+                                                      const int64_t* outer_offsets, int64_t outlength,
+                                                      int64_t maxcount) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < cm.n; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t e = cm.E0 + t;
+    const int64_t row = upper_bound(offsets, cm.row0, cm.row1 + 1, e) - 1;
+    const int64_t bin = upper_bound(outer_offsets, 0, outlength + 1, row) - 1;
+    const int64_t col = e - offsets[row];
+    cm.keys[t] = col < maxcount ? col : maxcount;
+    if (col < maxcount) atomicAdd((unsigned long long*)&cm.cnt[bin * maxcount + col], 1ull);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void segoff_kernel(ColMajor cm, const int64_t* offsets,
+                                                        const int64_t* outer_offsets, int64_t outlength) {
+  const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (b <= outlength) cm.segoff[b] = offsets[outer_offsets[b]] - cm.E0;
+}
+
+struct PrefixBody {
+  const int64_t* cnt;
+  int64_t* prefix;
+  unsigned long long* maxgroup;
+  __device__ int64_t count(int64_t g) const { return cnt[g]; }
+  __device__ void emit(int64_t g, int64_t excl, int64_t c) const {
+    if (g == 0) prefix[0] = 0;
+    prefix[g + 1] = excl + c;
+    if (c > 0) atomicMax(maxgroup, (unsigned long long)(g + 1));
+  }
+};
+
+// Fills every array of `cm` (host values included).  Arrays live in the
+// work-queue pool: the argsort and the scan use the scratch pool.
+Error colmajor(ColMajor& cm, const int64_t* offsets, const int64_t* outer_offsets, int64_t outlength,
+               int64_t maxcount) {
+  hipStream_t st = stream();
+  cm.row0 = read_device(outer_offsets, 0);
+  cm.row1 = read_device(outer_offsets, outlength);
+  cm.E0 = read_device(offsets, cm.row0);
+  cm.n = read_device(offsets, cm.row1) - cm.E0;
+  if (cm.n < 0) cm.n = 0;
+  cm.groups = outlength * maxcount;
+  const size_t nb = round256((size_t)cm.n * 8), sb = round256((size_t)(outlength + 1) * 8),
+               gb = round256((size_t)cm.groups * 8), pb = round256((size_t)(cm.groups + 1) * 8);
+  char* q = (char*)queue_pool(256 + 2 * nb + sb + gb + pb);
+  if (!q) return failure("out of device memory for the column regrouping", kSliceNone, kSliceNone, kFile);
+  cm.maxgroup = (unsigned long long*)q;
+  cm.keys = (int64_t*)(q + 256);
+  cm.order = (int64_t*)(q + 256 + nb);
+  cm.segoff = (int64_t*)(q + 256 + 2 * nb);
+  cm.cnt = (int64_t*)(q + 256 + 2 * nb + sb);
+  cm.prefix = (int64_t*)(q + 256 + 2 * nb + sb + gb);
+  set_i64<<<1, 1, 0, st>>>((int64_t*)cm.maxgroup, 0);
+  if (cm.groups > 0) fill_kernel<<<grid_for(cm.groups, kBlock, 8), kBlock, 0, st>>>(cm.cnt, cm.groups, 0);
+  segoff_kernel<<<(unsigned)ceil_div(outlength + 1, (int64_t)kBlock), kBlock, 0, st>>>(cm, offsets, outer_offsets,
+                                                                                       outlength);
+  if (cm.n > 0) {
+    keys_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, offsets, outer_offsets, outlength, maxcount);
+    AKHIP_LAUNCH_CHECK("keys_kernel");
+    Error e = awkward_argsort_int64_int64(cm.order, cm.keys, cm.n, cm.segoff, outlength + 1, true, true);
+    if (e.str) return e;
+  }
+  set_i64<<<1, 1, 0, st>>>(cm.prefix, 0);
+  if (cm.groups > 0) {
+    const int64_t ntiles = ceil_div(cm.groups, (int64_t)256 * 8);
+    void* s = scratch(scan_scratch_bytes(ntiles));
+    if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
+    Error e = launch_lookback_scan<256, 8>(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
+    if (e.str) return e;
+  }
+  AKHIP_LAUNCH_CHECK("colmajor");
+  return success();
+}
+
+// sorted position t -> (bin, element, column)
+struct Sorted {
+  int64_t bin, e, col, local;
+};
+__device__ inline Sorted sorted_at(const ColMajor& cm, int64_t t, int64_t outlength) {
+  const int64_t b = upper_bound(cm.segoff, 0, outlength + 1, t) - 1;
+  const int64_t local = cm.order[t];
+  const int64_t rel = cm.segoff[b] + local;
+  return Sorted{b, cm.E0 + rel, cm.keys[rel], t - cm.segoff[b]};
+}
+
+__global__ __launch_bounds__(kBlock) void place_kernel(ColMajor cm, int64_t* nextcarry, int64_t outlength,
+                                                       int64_t maxcount) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < cm.n; t += (int64_t)gridDim.x * kBlock) {
+    const Sorted s = sorted_at(cm, t, outlength);
+    if (s.col < maxcount) nextcarry[cm.prefix[s.bin * maxcount] + s.local] = s.e;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void groups_out_kernel(ColMajor cm, int64_t* nextoffsets, int64_t* distincts,
+                                                            int64_t distinctslen) {
+  const int64_t hi = cm.groups + 1 > distinctslen ? cm.groups + 1 : distinctslen;
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < hi; g += (int64_t)gridDim.x * kBlock) {
+    if (g <= cm.groups) nextoffsets[g] = cm.prefix[g];
+    if (g < distinctslen) distincts[g] = (g < cm.groups && cm.cnt[g] > 0) ? g : -1;
+  }
+}
+
+__global__ void maxgroup_out(const ColMajor cm, int64_t* maxnextparents) {
+  *maxnextparents = (int64_t)*cm.maxgroup - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void missing_kernel(ColMajor cm, int64_t* missing, const int64_t* offsets,
+                                                         const int64_t* outer_offsets, int64_t outlength,
+                                                         int64_t maxcount) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < cm.n; t += (int64_t)gridDim.x * kBlock) {
+    const Sorted s = sorted_at(cm, t, outlength);
+    if (s.col >= maxcount) continue;
+    const int64_t g0 = s.bin * maxcount;
+    const int64_t rank = s.local - (cm.prefix[g0 + s.col] - cm.prefix[g0]);
+    const int64_t row = upper_bound(offsets, cm.row0, cm.row1 + 1, s.e) - 1;
+    missing[s.e] = (row - outer_offsets[s.bin]) - rank;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gather_kernel(int64_t* out, const int64_t* src, const int64_t* index,
+                                                        int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    out[k] = src[index[k]];
+}
+
+// nummissing[k] after the CPU loop: rows of the last bin shorter than k + 1
+__global__ __launch_bounds__(kBlock) void short_rows_kernel(int64_t* hist, const int64_t* offsets, int64_t r0,
+                                                            int64_t r1, int64_t maxcount) {
+  for (int64_t r = r0 + (int64_t)blockIdx.x * kBlock + threadIdx.x; r < r1; r += (int64_t)gridDim.x * kBlock) {
+    const int64_t c = offsets[r + 1] - offsets[r];
+    if (c < maxcount) atomicAdd((unsigned long long*)&hist[c > 0 ? c : 0], 1ull);
+  }
+}
+__global__ void nummissing_kernel(int64_t* nummissing, const int64_t* hist, int64_t maxcount) {
+  int64_t run = 0;
+  for (int64_t k = 0; k < maxcount; k++) {
+    run += hist[k];
+    nummissing[k] = run;
+  }
+}
+
+}  // namespace
+}  // namespace akhip
+
+using namespace akhip;
+
+extern "C" {
+
+Error awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64(int64_t* maxcount, int64_t* offsetscopy,
+                                                                      const int64_t* offsets, int64_t length) {
+  hipStream_t st = stream();
+  set_i64<<<1, 1, 0, st>>>(maxcount, 0);
+  if (length <= 0) {
+    hipMemcpyAsync(offsetscopy, offsets, sizeof(int64_t), hipMemcpyDeviceToDevice, st);
+    return success();
+  }
+  maxcount_kernel<<<grid_for(length, kBlock, 8), kBlock, 0, st>>>((unsigned long long*)maxcount, offsetscopy, offsets,
+                                                                  length);
+  AKHIP_LAUNCH_CHECK("maxcount_kernel");
+  return success();
+}
+
+Error awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64(int64_t* outstarts, int64_t* outstops,
+                                                                const int64_t* distincts, int64_t lendistincts,
+                                                                int64_t outlength) {
+  if (outlength <= 0) return success();
+  hipStream_t st = stream();
+  const unsigned grid = (unsigned)ceil_div(outlength, (int64_t)kBlock);
+  if (lendistincts <= 0) {
+    fill_kernel<<<grid, kBlock, 0, st>>>(outstarts, outlength, 0);
+    fill_kernel<<<grid, kBlock, 0, st>>>(outstops, outlength, 0);
+  } else if (lendistincts < outlength) {
+    outstartsstops_serial<<<1, 1, 0, st>>>(outstarts, outstops, distincts, lendistincts, outlength);
+  } else {
+    outstartsstops_kernel<<<grid, kBlock, 0, st>>>(outstarts, outstops, distincts, lendistincts, outlength,
+                                                   lendistincts / outlength);
+  }
+  AKHIP_LAUNCH_CHECK("outstartsstops_kernel");
+  return success();
+}
+
+Error awkward_ListOffsetArray_reduce_nonlocal_preparenext_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t,
+                                                             int64_t* maxnextparents, int64_t* distincts,
+                                                             int64_t distinctslen, int64_t*, const int64_t* offsets,
+                                                             int64_t, const int64_t* outer_offsets, int64_t outlength,
+                                                             int64_t maxcount) {
+  hipStream_t st = stream();
+  if (outlength <= 0 || maxcount <= 0) {
+    set_i64<<<1, 1, 0, st>>>(maxnextparents, -1);
+    const int64_t groups = outlength > 0 && maxcount > 0 ? outlength * maxcount : 0;
+    fill_kernel<<<grid_for(groups + 1, kBlock, 8), kBlock, 0, st>>>(nextoffsets, groups + 1, 0);
+    if (distinctslen > 0) fill_kernel<<<grid_for(distinctslen, kBlock, 8), kBlock, 0, st>>>(distincts, distinctslen, -1);
+    AKHIP_LAUNCH_CHECK("preparenext (empty)");
+    return success();
+  }
+  ColMajor cm;
+  Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
+  if (e.str) return e;
+  if (cm.n > 0) place_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, nextcarry, outlength, maxcount);
+  const int64_t hi = cm.groups + 1 > distinctslen ? cm.groups + 1 : distinctslen;
+  groups_out_kernel<<<grid_for(hi, kBlock, 8), kBlock, 0, st>>>(cm, nextoffsets, distincts, distinctslen);
+  maxgroup_out<<<1, 1, 0, st>>>(cm, maxnextparents);
+  AKHIP_LAUNCH_CHECK("preparenext");
+  return success();
+}
+
+Error awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64(int64_t* nummissing, int64_t* missing,
+                                                            int64_t* nextshifts, const int64_t* offsets, int64_t,
+                                                            const int64_t*, const int64_t* outer_offsets,
+                                                            int64_t outlength, int64_t maxcount, int64_t nextlen,
+                                                            const int64_t* nextcarry) {
+  if (outlength <= 0) return success();  // the CPU loop writes nothing
+  hipStream_t st = stream();
+  if (maxcount > 0) {
+    ColMajor cm;
+    Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
+    if (e.str) return e;
+    if (cm.n > 0)
+      missing_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, missing, offsets, outer_offsets, outlength,
+                                                                   maxcount);
+    // nummissing: the last bin's short rows (histogram by length, then a running sum)
+    int64_t* hist = cm.cnt;  // groups >= maxcount entries, free again after the scan
+    fill_kernel<<<grid_for(maxcount, kBlock, 8), kBlock, 0, st>>>(hist, maxcount, 0);
+    const int64_t r0 = read_device(outer_offsets, outlength - 1), r1 = cm.row1;
+    if (r1 > r0) short_rows_kernel<<<grid_for(r1 - r0, kBlock, 8), kBlock, 0, st>>>(hist, offsets, r0, r1, maxcount);
+    nummissing_kernel<<<1, 1, 0, st>>>(nummissing, hist, maxcount);
+  }
+  if (nextlen > 0) gather_kernel<<<grid_for(nextlen, kBlock, 8), kBlock, 0, st>>>(nextshifts, missing, nextcarry, nextlen);
+  AKHIP_LAUNCH_CHECK("nextshifts");
+  return success();
+}
+
+}  // extern "C"

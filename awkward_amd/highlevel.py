@@ -13,7 +13,9 @@ functions the configs use, with the reference's signatures and defaults:
   index, or ``[:, start:stop:step]``; elementwise ufuncs on lists that share
   offsets.
 
-All of them run at axis=-1 (the innermost list level) of one level of lists.
+The reducers take any axis of a list array (axis=-1 is the local path;
+higher axes reduce across lists by position, through option types and
+missing lists); the other operations run at axis=-1 of one level of lists.
 """
 
 from __future__ import annotations
@@ -223,8 +225,11 @@ def _check_axis(axis):
 
 
 def _reduce(array, reducer, keepdims, mask_identity, axis):
-    _check_axis(axis)
-    return Array(C.reduce(array.layout, reducer, mask_identity, keepdims))
+    """Any axis of a list array (negative counts from the numbers); axis=-1
+    is the local path, higher axes reduce across lists by position."""
+    if axis is None:
+        raise NotImplementedError("axis=None (a full reduction) is not on the hip jagged-array path")
+    return Array(C.reduce(array.layout, reducer, mask_identity, keepdims, axis))
 
 
 def sum(array, axis=-1, keepdims=False, mask_identity=False):
@@ -319,35 +324,37 @@ def from_offsets(offsets, content, backend):
     return Array(C.ListOffsetArray(off, inner, backend))
 
 
+def _layout_from_python(items, backend, dtype):
+    """One nesting level of Python values as a layout: lists become a
+    ListOffsetArray over the next level, dicts a RecordArray of fields,
+    numbers a NumpyArray; None makes the level an option type,
+    IndexedOptionArray(int64 index, -1 for None) over the present values --
+    the layouts the reference's ak.from_iter builds."""
+    nplike = backend.nplike
+    if any(v is None for v in items):
+        present = [v for v in items if v is not None]
+        index = np.full(len(items), -1, dtype=np.int64)
+        index[[i for i, v in enumerate(items) if v is not None]] = np.arange(len(present))
+        return C.IndexedOptionArray(nplike.asarray(index), _layout_from_python(present, backend, dtype), backend)
+    if items and isinstance(items[0], (list, tuple)):
+        counts = [len(x) for x in items]
+        offsets = np.zeros(len(items) + 1, dtype=np.int64)
+        np.cumsum(counts, out=offsets[1:])
+        inner = _layout_from_python([v for x in items for v in x], backend, dtype)
+        return C.ListOffsetArray(nplike.asarray(offsets), inner, backend)
+    if items and isinstance(items[0], dict):
+        keys = list(items[0])
+        cols = [C.NumpyArray(nplike.asarray(np.ascontiguousarray(np.asarray([r[k] for r in items], dtype=dtype))),
+                             backend) for k in keys]
+        return C.RecordArray(cols, keys, len(items), backend)
+    values = np.asarray(items, dtype=dtype if dtype is not None else (np.float64 if not items else None))
+    return C.NumpyArray(nplike.asarray(np.ascontiguousarray(values)), backend)
+
+
 def from_list(lists, backend, dtype=None):
-    """A list of lists of numbers (or of dicts of numbers) as an Array.
-    None among the numbers makes an option type: IndexedOptionArray(int64
-    index, -1 for None) over the non-None values, the layout the reference's
-    ak.from_iter builds for optional numbers."""
-    if any(x is None for x in lists):  # missing lists: an option type over the lists
-        present = [x for x in lists if x is not None]
-        index = np.full(len(lists), -1, dtype=np.int64)
-        index[[i for i, x in enumerate(lists) if x is not None]] = np.arange(len(present))
-        inner = from_list(present, backend, dtype).layout
-        return Array(C.IndexedOptionArray(backend.nplike.asarray(index), inner, backend))
-    counts = [len(x) for x in lists]
-    offsets = np.zeros(len(lists) + 1, dtype=np.int64)
-    np.cumsum(counts, out=offsets[1:])
-    flat = [v for x in lists for v in x]
-    if any(v is None for v in flat):
-        present = [v for v in flat if v is not None]
-        index = np.full(len(flat), -1, dtype=np.int64)
-        index[[i for i, v in enumerate(flat) if v is not None]] = np.arange(len(present))
-        values = np.asarray(present, dtype=dtype if dtype is not None else (np.float64 if not present else None))
-        nplike = backend.nplike
-        inner = C.IndexedOptionArray(nplike.asarray(index), C.NumpyArray(nplike.asarray(values), backend), backend)
-        return Array(C.ListOffsetArray(nplike.asarray(offsets), inner, backend))
-    if flat and isinstance(flat[0], dict):
-        keys = list(flat[0])
-        content = {k: np.asarray([r[k] for r in flat], dtype=dtype) for k in keys}
-    else:
-        content = np.asarray(flat, dtype=dtype if dtype is not None else (np.float64 if not flat else None))
-    return from_offsets(offsets, content, backend)
+    """Nested Python lists (of numbers, dicts of numbers, or None) as an
+    Array (ak.from_iter's layouts: see _layout_from_python)."""
+    return Array(_layout_from_python(list(lists), backend, dtype))
 
 
 def to_backend(array, backend):

@@ -492,6 +492,22 @@ struct Error awkward_IndexedArrayU32_getitem_nextcarry_outindex_64(int64_t* toca
 /* replaces awkward-cpp/src/cpu-kernels/awkward_NumpyArray_reduce_adjust_starts_shifts_64.cpp:3; contract kernel-specification.yml:2566 */
 struct Error awkward_NumpyArray_reduce_adjust_starts_shifts_64(int64_t* toptr, int64_t outlength, const int64_t* offsets, const int64_t* starts, const int64_t* shifts);
 
+/* ---- awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64.cpp:3; contract kernel-specification.yml:2197 */
+struct Error awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64(int64_t* maxcount, int64_t* offsetscopy, const int64_t* offsets, int64_t length);
+
+/* ---- awkward_ListOffsetArray_reduce_nonlocal_preparenext_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ListOffsetArray_reduce_nonlocal_preparenext_64.cpp:3; contract kernel-specification.yml:2296 */
+struct Error awkward_ListOffsetArray_reduce_nonlocal_preparenext_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t nextlen, int64_t* maxnextparents, int64_t* distincts, int64_t distinctslen, int64_t* offsetscopy, const int64_t* offsets, int64_t length, const int64_t* outer_offsets, int64_t outlength, int64_t maxcount);
+
+/* ---- awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64.cpp:3; contract kernel-specification.yml:2219 */
+struct Error awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64(int64_t* nummissing, int64_t* missing, int64_t* nextshifts, const int64_t* offsets, int64_t length, const int64_t* starts, const int64_t* outer_offsets, int64_t outlength, int64_t maxcount, int64_t nextlen, const int64_t* nextcarry);
+
+/* ---- awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64 ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64.cpp:6; contract kernel-specification.yml:2256 */
+struct Error awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64(int64_t* outstarts, int64_t* outstops, const int64_t* distincts, int64_t lendistincts, int64_t outlength);
+
 /* ---- runtime (no counterpart in awkward-cpp, which is synchronous CPU code) ---- */
 
 /* Stream that subsequent entry points enqueue on (a hipStream_t; NULL = the

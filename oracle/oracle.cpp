@@ -684,6 +684,97 @@ Error awkward_NumpyArray_reduce_adjust_starts_shifts_64(int64_t* toptr, int64_t 
   return ok();
 }
 
+// ---- non-local (axis != -1) reductions ------------------------------------------
+// awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64.cpp
+Error awkward_ListOffsetArray_reduce_nonlocal_maxcount_offsetscopy_64(int64_t* maxcount, int64_t* offsetscopy,
+                                                                      const int64_t* offsets, int64_t length) {
+  *maxcount = 0;
+  offsetscopy[0] = offsets[0];
+  for (int64_t i = 0; i < length; i++) {
+    const int64_t count = offsets[i + 1] - offsets[i];
+    if (*maxcount < count) *maxcount = count;
+    offsetscopy[i + 1] = offsets[i + 1];
+  }
+  return ok();
+}
+
+// awkward_ListOffsetArray_reduce_nonlocal_preparenext_64.cpp: the elements
+// of each outer bin regrouped by column (position in their row), rows in
+// order within a column; a group per (bin, column), nextbin = bin *
+// maxcount + column.
+Error awkward_ListOffsetArray_reduce_nonlocal_preparenext_64(int64_t* nextcarry, int64_t* nextoffsets, int64_t,
+                                                             int64_t* maxnextparents, int64_t* distincts,
+                                                             int64_t distinctslen, int64_t*, const int64_t* offsets,
+                                                             int64_t, const int64_t* outer_offsets, int64_t outlength,
+                                                             int64_t maxcount) {
+  *maxnextparents = -1;
+  for (int64_t i = 0; i < distinctslen; i++) distincts[i] = -1;
+  int64_t k = 0;
+  nextoffsets[0] = 0;
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    for (int64_t col = 0; col < maxcount; col++) {
+      const int64_t nextbin = bin * maxcount + col;
+      bool any = false;
+      for (int64_t row = outer_offsets[bin]; row < outer_offsets[bin + 1]; row++) {
+        if (col < offsets[row + 1] - offsets[row]) {
+          nextcarry[k++] = offsets[row] + col;
+          any = true;
+        }
+      }
+      nextoffsets[nextbin + 1] = k;
+      if (any) {
+        if (nextbin < distinctslen) distincts[nextbin] = nextbin;
+        if (nextbin > *maxnextparents) *maxnextparents = nextbin;
+      }
+    }
+  }
+  return ok();
+}
+
+// awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64.cpp: missing[e] for
+// the element in column j of a row is the number of earlier rows of its bin
+// that are too short to have a column j; nextshifts gathers it through
+// nextcarry; nummissing is left as the last bin's per-column counts.
+Error awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64(int64_t* nummissing, int64_t* missing,
+                                                            int64_t* nextshifts, const int64_t* offsets, int64_t,
+                                                            const int64_t*, const int64_t* outer_offsets,
+                                                            int64_t outlength, int64_t maxcount, int64_t nextlen,
+                                                            const int64_t* nextcarry) {
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    for (int64_t k = 0; k < maxcount; k++) nummissing[k] = 0;
+    for (int64_t i = outer_offsets[bin]; i < outer_offsets[bin + 1]; i++) {
+      const int64_t start = offsets[i], count = offsets[i + 1] - offsets[i];
+      for (int64_t k = count; k < maxcount; k++) nummissing[k]++;
+      for (int64_t j = 0; j < count; j++) missing[start + j] = nummissing[j];
+    }
+  }
+  for (int64_t j = 0; j < nextlen; j++) nextshifts[j] = missing[nextcarry[j]];
+  return ok();
+}
+
+// awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64.cpp: each outer
+// bin's output list spans its columns up to the last one that had data.
+Error awkward_ListOffsetArray_reduce_nonlocal_outstartsstops_64(int64_t* outstarts, int64_t* outstops,
+                                                                const int64_t* distincts, int64_t lendistincts,
+                                                                int64_t outlength) {
+  if (outlength > 0 && lendistincts > 0) {
+    const int64_t maxcount = lendistincts / outlength;
+    int64_t k = 0, i_next = 0;
+    for (int64_t i = 0; i < lendistincts; i++) {
+      if (i == i_next) {
+        i_next += maxcount;
+        outstarts[k] = i;
+        outstops[k] = i;
+        k++;
+      }
+      if (distincts[i] != -1) outstops[k - 1] = i + 1;
+    }
+  } else {
+    for (int64_t k = 0; k < outlength; k++) outstarts[k] = outstops[k] = 0;
+  }
+  return ok();
+}
+
 }  // extern "C"
 
 namespace {

@@ -22,7 +22,7 @@ def test_header_declares_every_specialization():
     names = set(declared())
     missing = [n for n in SPECIALIZATIONS if n not in names]
     assert not missing, missing
-    assert len(SPECIALIZATIONS) == 185  # 163 on the jagged path + 22 for option-type contents
+    assert len(SPECIALIZATIONS) == 189  # 163 on the jagged path, 22 option-type, 4 non-local
 
 
 @pytest.mark.skipif(not os.path.exists(LIBPATH), reason="libawkward_hip.so not built")
