@@ -235,12 +235,14 @@ Error batch_argsort(const BatchSortItem* items, int32_t nitems, int64_t total_ti
   const int64_t small_capacity = small_queue_capacity(ne);
   const size_t qb = ((size_t)capacity * sizeof(LongChunk) + 255) / 256 * 256;
   const size_t sb = ((size_t)small_capacity * sizeof(LongChunk) + 255) / 256 * 256;
-  char* s = (char*)scratch(256 + qb + sb + 2 * (kb + ib));
+  const size_t cb = corank_bytes(capacity);
+  char* s = (char*)scratch(256 + qb + sb + cb + 2 * (kb + ib));
   if (!s) return failure("out of device memory for batch sort scratch", kSliceNone, kSliceNone,
                          "awkward_amd/csrc/batch.hip");
   LongQueue q{(LongChunk*)(s + 256), (unsigned long long*)s, (unsigned long long*)(s + 8), capacity,
-              (LongChunk*)(s + 256 + qb), (unsigned long long*)(s + 16), small_capacity};
-  char* b = s + 256 + qb + sb;
+              (LongChunk*)(s + 256 + qb), (unsigned long long*)(s + 16), small_capacity,
+              (int64_t*)(s + 256 + qb + sb)};
+  char* b = s + 256 + qb + sb + cb;
   hipStream_t st = stream();
   long_queue_reset<><<<1, 1, 0, st>>>(q.count, q.maxlen, q.small_count);
   int64_t grid = (int64_t)cu_count() * 4;

@@ -286,7 +286,11 @@ struct LongQueue {
   LongChunk* small;
   unsigned long long* small_count;
   int64_t small_capacity;
+  int64_t* coranks;            // 2 per chunk: a merge pass's split of its output range
 };
+
+// Bytes of the co-rank array for a chunk queue of `capacity` entries.
+inline size_t corank_bytes(int64_t capacity) { return ((size_t)capacity * 16 + 255) / 256 * 256; }
 
 // Called by every lane of a wave: the lanes with want set append their
 // segment -- one entry in the small queue, or its chunks in the chunk queue;
@@ -467,9 +471,52 @@ __device__ inline void corank_pair(const uint64_t* ka, const uint32_t* ia, int64
   __syncthreads();
 }
 
+// Before each merge pass: the co-ranks of every chunk's two output
+// boundaries (one thread per boundary, a binary search in global memory --
+// all of them in flight at once, where a merging workgroup would wait on
+// them one chunk after another).  P(i) = "A[i] sorts before B[d - i - 1]"
+// is true below the co-rank and false from it on.
+template <int Dummy = 0>
+__global__ __launch_bounds__(kBlock) void long_corank_kernel(const LongChunk* __restrict__ chunks,
+                                                             const unsigned long long* __restrict__ count,
+                                                             const unsigned long long* __restrict__ maxlen,
+                                                             int64_t capacity, int64_t w,
+                                                             const uint64_t* __restrict__ sk,
+                                                             const uint32_t* __restrict__ si,
+                                                             int64_t* __restrict__ coranks) {
+  if ((int64_t)*maxlen <= w) return;
+  int64_t nc = (int64_t)*count;
+  if (nc > capacity) nc = capacity;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nc; t += (int64_t)gridDim.x * kBlock) {
+    const LongChunk ch = chunks[t >> 1];
+    if (ch.len <= w) continue;
+    const int64_t d0 = ch.chunk * kCap;
+    const int64_t d = (t & 1) ? ((d0 + kCap) < ch.len ? d0 + kCap : ch.len) : d0;
+    const int64_t ps = d0 / (2 * w) * (2 * w);
+    const int64_t na = (ch.len - ps) < w ? ch.len - ps : w;
+    const int64_t rest = ch.len - ps - na;
+    const int64_t nb = rest < w ? rest : w;
+    const uint64_t* ka = sk + ch.buf + ps;
+    const uint32_t* ia = si + ch.buf + ps;
+    const uint64_t* kb = ka + na;
+    const uint32_t* ib = ia + na;
+    const int64_t dd = d - ps;
+    int64_t lo = dd > nb ? dd - nb : 0;
+    int64_t hi = dd < na ? dd : na;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int64_t j = dd - mid;
+      if (j > 0 && pair_less(ka[mid], ia[mid], kb[j - 1], ib[j - 1])) lo = mid + 1;
+      else hi = mid;
+    }
+    coranks[t] = lo;
+  }
+}
+
 // Merge pass over runs of width w: the workgroup for chunk c produces
 // elements [c * kCap, (c + 1) * kCap) of the merged pair of runs holding
-// them; the pass that completes a segment writes its output.
+// them, between the co-ranks long_corank_kernel found; the pass that
+// completes a segment writes its output.
 template <typename T, bool ARG>
 __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __restrict__ chunks,
                                                             const unsigned long long* __restrict__ count,
@@ -477,13 +524,12 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
                                                             int64_t capacity, int64_t w,
                                                             const uint64_t* __restrict__ sk,
                                                             const uint32_t* __restrict__ si, uint64_t* __restrict__ dk,
-                                                            uint32_t* __restrict__ di) {
+                                                            uint32_t* __restrict__ di,
+                                                            const int64_t* __restrict__ coranks) {
   if ((int64_t)*maxlen <= w) return;  // every queued segment is already one run
   constexpr int PER = kCap / kBlock;   // outputs per thread
   __shared__ uint64_t s_key[kCap];
   __shared__ uint32_t s_idx[kCap];
-  __shared__ int64_t s_diag[2], s_co[2];
-  __shared__ int s_cnt[kBlock / kWave];
   int64_t nc = (int64_t)*count;
   if (nc > capacity) nc = capacity;
   const int tid = threadIdx.x;
@@ -494,19 +540,12 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     const int64_t d1 = (d0 + kCap) < ch.len ? d0 + kCap : ch.len;
     const int64_t ps = d0 / (2 * w) * (2 * w);
     const int64_t na = (ch.len - ps) < w ? ch.len - ps : w;
-    const int64_t rest = ch.len - ps - na;
-    const int64_t nb = rest < w ? rest : w;
     const uint64_t* ka = sk + ch.buf + ps;
     const uint32_t* ia = si + ch.buf + ps;
     const uint64_t* kb = ka + na;
     const uint32_t* ib = ia + na;
-    if (tid == 0) {
-      s_diag[0] = d0 - ps;
-      s_diag[1] = d1 - ps;
-    }
-    __syncthreads();
-    corank_pair(ka, ia, na, kb, ib, nb, s_diag, s_co, s_cnt);
-    const int64_t x0 = s_co[0], x1 = s_co[1];
+    const int64_t x0 = coranks[2 * c], x1 = coranks[2 * c + 1];
+    const int64_t s_diag[2] = {d0 - ps, d1 - ps};
     const int64_t y0 = s_diag[0] - x0, y1 = s_diag[1] - x1;
     const int nal = (int)(x1 - x0), nbl = (int)(y1 - y0);
     const int n = nal + nbl;
@@ -579,9 +618,14 @@ inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc
   long_runs_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q, asc, gk0, gi0);
   uint64_t *sk = gk0, *dk = gk1;
   uint32_t *si = gi0, *di = gi1;
+  int64_t cgrid = (int64_t)cu_count() * 16;
+  if (cgrid > (2 * q.capacity + kBlock - 1) / kBlock) cgrid = (2 * q.capacity + kBlock - 1) / kBlock;
+  if (cgrid < 1) cgrid = 1;
   for (int64_t w = kCap; w < max_len; w <<= 1) {
+    long_corank_kernel<><<<(unsigned)cgrid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
+                                                             q.coranks);
     long_merge_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
-                                                                 dk, di);
+                                                                 dk, di, q.coranks);
     uint64_t* tk = sk;
     sk = dk;
     dk = tk;

@@ -58,7 +58,8 @@ inline bool sort_scratch(int64_t length, SortScratch* out) {
   const int64_t small_capacity = small_queue_capacity(length);
   const size_t qbytes = ((size_t)capacity * sizeof(LongChunk) + 255) / 256 * 256;
   const size_t sbytes = ((size_t)small_capacity * sizeof(LongChunk) + 255) / 256 * 256;
-  char* s = (char*)scratch(512 + qbytes + sbytes + 2 * (kbytes + ibytes));
+  const size_t cbytes = corank_bytes(capacity);
+  char* s = (char*)scratch(512 + qbytes + sbytes + cbytes + 2 * (kbytes + ibytes));
   if (!s) return false;
   out->head = s;
   out->q.count = (unsigned long long*)(s + 256);
@@ -68,7 +69,8 @@ inline bool sort_scratch(int64_t length, SortScratch* out) {
   out->q.capacity = capacity;
   out->q.small = (LongChunk*)(s + 512 + qbytes);
   out->q.small_capacity = small_capacity;
-  char* b = s + 512 + qbytes + sbytes;
+  out->q.coranks = (int64_t*)(s + 512 + qbytes + sbytes);
+  char* b = s + 512 + qbytes + sbytes + cbytes;
   out->gk0 = (uint64_t*)b;
   out->gk1 = (uint64_t*)(b + kbytes);
   out->gi0 = (uint32_t*)(b + 2 * kbytes);
