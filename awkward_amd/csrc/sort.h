@@ -37,6 +37,12 @@ constexpr int kCap = 4096;
 constexpr uint64_t kPadKey = ~uint64_t(0);
 constexpr uint32_t kPadIdx = ~uint32_t(0);
 
+// Order keys of types of at most 4 bytes fit 32 bits: the LDS stages and
+// merge buffers of the long-segment path carry them as uint32 (8 bytes per
+// (key, index) pair instead of 12).
+template <typename T>
+using KeyOf = std::conditional_t<(sizeof(T) <= 4), uint32_t, uint64_t>;
+
 template <typename T>
 __device__ inline uint64_t order_key(T v, bool ascending) {
   if constexpr (std::is_same_v<T, bool>) {
@@ -65,7 +71,8 @@ __device__ inline uint64_t order_key(T v, bool ascending) {
   }
 }
 
-__device__ inline bool pair_less(uint64_t ka, uint32_t ia, uint64_t kb, uint32_t ib) {
+template <typename K>
+__device__ inline bool pair_less(K ka, uint32_t ia, K kb, uint32_t ib) {
   return ka < kb || (ka == kb && ia < ib);
 }
 
@@ -78,7 +85,8 @@ __device__ inline void store_sorted(void* out, const T* from, int64_t a, int64_t
 
 // ---- len <= 16: registers ------------------------------------------------------
 // 16 (key, index) pairs in registers into ascending order (bitonic network).
-__device__ inline void bitonic16(uint64_t* k, uint32_t* ix) {
+template <typename K>
+__device__ inline void bitonic16(K* k, uint32_t* ix) {
 #pragma unroll
   for (int kk = 2; kk <= 16; kk <<= 1) {
 #pragma unroll
@@ -90,7 +98,7 @@ __device__ inline void bitonic16(uint64_t* k, uint32_t* ix) {
           const bool up = (i & kk) == 0;
           const bool swap = up ? pair_less(k[l], ix[l], k[i], ix[i]) : pair_less(k[i], ix[i], k[l], ix[l]);
           if (swap) {
-            const uint64_t tk = k[i];
+            const K tk = k[i];
             k[i] = k[l];
             k[l] = tk;
             const uint32_t ti = ix[i];
@@ -163,8 +171,8 @@ constexpr int kCapPad = kCap + kCap / 16;
 // >= np (the merge stages and their barriers run up to np_sync), so several
 // groups of threads may sort different segments side by side.  The sorted
 // pairs end in s_key / s_idx at positions pad16(0 .. n-1).
-template <typename T, int THREADS>
-__device__ void block_sort_part(uint64_t* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
+template <typename T, int THREADS, typename K = KeyOf<T>>
+__device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
                                 uint32_t idx_base, int t, int np_sync) {
   constexpr int PER = 16;
   int np = PER;
@@ -172,16 +180,16 @@ __device__ void block_sort_part(uint64_t* s_key, uint32_t* s_idx, const T* from,
   for (int i = t; i < np; i += THREADS) {
     const int p = pad16(i);
     if (i < n) {
-      s_key[p] = order_key<T>(from[a + i], asc);
+      s_key[p] = (K)order_key<T>(from[a + i], asc);
       s_idx[p] = idx_base + (uint32_t)i;
     } else {
-      s_key[p] = kPadKey;
+      s_key[p] = ~K(0);
       s_idx[p] = kPadIdx;
     }
   }
   __syncthreads();
   const bool active = t * PER < np;
-  uint64_t k[PER];
+  K k[PER];
   uint32_t x[PER];
   const int o = t * PER;  // this thread's first position
   if (active) {
@@ -218,7 +226,7 @@ __device__ void block_sort_part(uint64_t* s_key, uint32_t* s_idx, const T* from,
       for (int u = 0; u < PER; u++) {
         const int pa = pad16(A0 + (xa < r ? xa : r - 1));
         const int pb = pad16(B0 + (yb < r ? yb : r - 1));
-        const uint64_t ka = s_key[pa], kb = s_key[pb];
+        const K ka = s_key[pa], kb = s_key[pb];
         const uint32_t ia = s_idx[pa], ib = s_idx[pb];
         const bool from_a = yb >= r || (xa < r && !pair_less(kb, ib, ka, ia));
         k[u] = from_a ? ka : kb;
@@ -242,7 +250,8 @@ __device__ void block_sort_part(uint64_t* s_key, uint32_t* s_idx, const T* from,
 // ---- longer: LDS runs + merge-path passes ------------------------------------------------
 // Number of elements taken from run A among the first `diag` outputs of the
 // stable merge of A (first) and B.
-__device__ inline int64_t merge_corank(const uint64_t* ka, const uint32_t* ia, int64_t na, const uint64_t* kb,
+template <typename K>
+__device__ inline int64_t merge_corank(const K* ka, const uint32_t* ia, int64_t na, const K* kb,
                                        const uint32_t* ib, int64_t nb, int64_t diag) {
   int64_t lo = diag > nb ? diag - nb : 0;
   int64_t hi = diag < na ? diag : na;
@@ -377,11 +386,11 @@ __global__ void long_queue_reset(unsigned long long* count, unsigned long long* 
 // one chunk straight to its output, longer ones into merge buffer 0
 // (segment-relative positions).
 template <typename T, bool ARG>
-__global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc, uint64_t* __restrict__ gk,
+__global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc, KeyOf<T>* __restrict__ gk,
                                                            uint32_t* __restrict__ gi) {
   constexpr int NW = kBlock / kWave;
   constexpr int QUARTER = kCapPad / NW;  // pad16(kSmall) fits
-  __shared__ uint64_t s_key[kCapPad];
+  __shared__ KeyOf<T> s_key[kCapPad];
   __shared__ uint32_t s_idx[kCapPad];
   __shared__ int s_np[NW];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -434,7 +443,8 @@ __global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc
 // remaining range, so a range of n takes about log_128(n) rounds of
 // parallel loads.  P(i) = "A[i] sorts before B[diag - i - 1]" is true below
 // the co-rank and false from it on.
-__device__ inline void corank_pair(const uint64_t* ka, const uint32_t* ia, int64_t na, const uint64_t* kb,
+template <typename K>
+__device__ inline void corank_pair(const K* ka, const uint32_t* ia, int64_t na, const K* kb,
                                    const uint32_t* ib, int64_t nb, const int64_t* diag, int64_t* result,
                                    int* s_cnt) {
   constexpr int G = kBlock / 2;
@@ -476,12 +486,12 @@ __device__ inline void corank_pair(const uint64_t* ka, const uint32_t* ia, int64
 // all of them in flight at once, where a merging workgroup would wait on
 // them one chunk after another).  P(i) = "A[i] sorts before B[d - i - 1]"
 // is true below the co-rank and false from it on.
-template <int Dummy = 0>
+template <typename K>
 __global__ __launch_bounds__(kBlock) void long_corank_kernel(const LongChunk* __restrict__ chunks,
                                                              const unsigned long long* __restrict__ count,
                                                              const unsigned long long* __restrict__ maxlen,
                                                              int64_t capacity, int64_t w,
-                                                             const uint64_t* __restrict__ sk,
+                                                             const K* __restrict__ sk,
                                                              const uint32_t* __restrict__ si,
                                                              int64_t* __restrict__ coranks) {
   if ((int64_t)*maxlen <= w) return;
@@ -496,9 +506,9 @@ __global__ __launch_bounds__(kBlock) void long_corank_kernel(const LongChunk* __
     const int64_t na = (ch.len - ps) < w ? ch.len - ps : w;
     const int64_t rest = ch.len - ps - na;
     const int64_t nb = rest < w ? rest : w;
-    const uint64_t* ka = sk + ch.buf + ps;
+    const K* ka = sk + ch.buf + ps;
     const uint32_t* ia = si + ch.buf + ps;
-    const uint64_t* kb = ka + na;
+    const K* kb = ka + na;
     const uint32_t* ib = ia + na;
     const int64_t dd = d - ps;
     int64_t lo = dd > nb ? dd - nb : 0;
@@ -522,13 +532,14 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
                                                             const unsigned long long* __restrict__ count,
                                                             const unsigned long long* __restrict__ maxlen,
                                                             int64_t capacity, int64_t w,
-                                                            const uint64_t* __restrict__ sk,
-                                                            const uint32_t* __restrict__ si, uint64_t* __restrict__ dk,
-                                                            uint32_t* __restrict__ di,
+                                                            const KeyOf<T>* __restrict__ sk,
+                                                            const uint32_t* __restrict__ si,
+                                                            KeyOf<T>* __restrict__ dk, uint32_t* __restrict__ di,
                                                             const int64_t* __restrict__ coranks) {
+  using K = KeyOf<T>;
   if ((int64_t)*maxlen <= w) return;  // every queued segment is already one run
   constexpr int PER = kCap / kBlock;   // outputs per thread
-  __shared__ uint64_t s_key[kCap];
+  __shared__ K s_key[kCap];
   __shared__ uint32_t s_idx[kCap];
   int64_t nc = (int64_t)*count;
   if (nc > capacity) nc = capacity;
@@ -540,9 +551,9 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     const int64_t d1 = (d0 + kCap) < ch.len ? d0 + kCap : ch.len;
     const int64_t ps = d0 / (2 * w) * (2 * w);
     const int64_t na = (ch.len - ps) < w ? ch.len - ps : w;
-    const uint64_t* ka = sk + ch.buf + ps;
+    const K* ka = sk + ch.buf + ps;
     const uint32_t* ia = si + ch.buf + ps;
-    const uint64_t* kb = ka + na;
+    const K* kb = ka + na;
     const uint32_t* ib = ia + na;
     const int64_t x0 = coranks[2 * c], x1 = coranks[2 * c + 1];
     const int64_t s_diag[2] = {d0 - ps, d1 - ps};
@@ -560,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     }
     __syncthreads();
     // this thread's outputs [tid * PER, tid * PER + PER) of the block
-    uint64_t rk[PER];
+    K rk[PER];
     uint32_t ri[PER];
     const int o0 = tid * PER < n ? tid * PER : n;
     const int o1 = o0 + PER < n ? o0 + PER : n;
@@ -609,24 +620,27 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
 // (an upper bound of the longest segment known on the host).  Buffers
 // (gk0, gi0) and (gk1, gi1) cover the merge-buffer positions.
 template <typename T, bool ARG>
-inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc, uint64_t* gk0, uint32_t* gi0,
-                                   uint64_t* gk1, uint32_t* gi1, hipStream_t st) {
+inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc, void* gk0v, uint32_t* gi0,
+                                   void* gk1v, uint32_t* gi1, hipStream_t st) {
+  using K = KeyOf<T>;
+  K* gk0 = (K*)gk0v;
+  K* gk1 = (K*)gk1v;
   if (max_len <= kWave) return hipSuccess;
   int64_t grid = (int64_t)cu_count() * 8;
   if (grid > q.capacity + q.small_capacity) grid = q.capacity + q.small_capacity;
   if (grid < 1) grid = 1;
   long_runs_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q, asc, gk0, gi0);
-  uint64_t *sk = gk0, *dk = gk1;
+  K *sk = gk0, *dk = gk1;
   uint32_t *si = gi0, *di = gi1;
   int64_t cgrid = (int64_t)cu_count() * 16;
   if (cgrid > (2 * q.capacity + kBlock - 1) / kBlock) cgrid = (2 * q.capacity + kBlock - 1) / kBlock;
   if (cgrid < 1) cgrid = 1;
   for (int64_t w = kCap; w < max_len; w <<= 1) {
-    long_corank_kernel<><<<(unsigned)cgrid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
+    long_corank_kernel<K><<<(unsigned)cgrid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
                                                              q.coranks);
     long_merge_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
                                                                  dk, di, q.coranks);
-    uint64_t* tk = sk;
+    K* tk = sk;
     sk = dk;
     dk = tk;
     uint32_t* ti = si;

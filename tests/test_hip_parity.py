@@ -386,6 +386,23 @@ def test_long_segments(t, ascending):
     assert same_bits(h[0], d[0])
 
 
+@pytest.mark.parametrize("t", ["bool", "int8", "uint16", "uint32"])
+def test_long_segments_narrow_keys(t):
+    """Long segments of the types whose order keys travel as 32 bits (every
+    type of at most 4 bytes), both directions, with many ties."""
+    rng = np.random.default_rng(27)
+    counts = np.array([9000, 4097, 70000, 12288, 5])
+    off = offsets_from_counts(counts)
+    x = sort_values(rng, t, int(off[-1]))
+    for ascending in (True, False):
+        h, d, _, _ = run_both(f"awkward_argsort_{t}_int64", [np.zeros(len(x), np.int64), x, len(x), off, len(off),
+                                                             ascending, True])
+        assert same_bits(h[0], d[0]), ascending
+        h, d, _, _ = run_both(f"awkward_sort_{t}", [np.zeros(len(x), x.dtype), x, len(x), off, len(off), ascending,
+                                                    True])
+        assert same_bits(h[0], d[0]), ascending
+
+
 @pytest.mark.parametrize("variant", range(10))
 @pytest.mark.parametrize("t", ["int32", "int64", "float32", "float64"])
 def test_tuned_variants(variant, t):
