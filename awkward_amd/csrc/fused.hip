@@ -255,12 +255,44 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
   const int red_slot = st.slot[red];
   const float* red_lds = (staged && red_slot >= 0) ? s_stage + red_slot * st.cap : nullptr;
   const float* red_g = (const float*)fields.ptr[red];
+  // [start:] with step 1 (start >= 0, or omitted): the kept particles are the
+  // selected ones of rank >= start, known without the count -- one pass
+  const int64_t start0 = start == kSliceNone ? 0 : start;
+  const bool one_pass = staged && step == 1 && stop == kSliceNone && start0 >= 0;
 #pragma unroll
   for (int j = 0; j < LPT; j++) {
     const int e = tid + j * BLOCK;
     if (e >= nl) break;
     const int64_t ev = L0 + e;
     const int64_t a = s_off[e], b = s_off[e + 1];
+    if (one_pass) {
+      float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
+      int64_t best = -1, t = 0, sel = 0;
+      for (int q = (int)(a - E0); q < (int)(b - E0); q++) {
+        if (!s_flag[q]) continue;
+        if (sel >= start0) {
+          const float v = red_lds ? red_lds[q] : red_g[E0 + q];
+          sum += v;
+          if (t == 0) {
+            best = 0;
+            best_val = v;
+          } else if (v > best_val) {
+            best_val = v;
+            best = t;
+          }
+          maxv = (v > maxv) ? v : maxv;
+          t++;
+        }
+        sel++;
+      }
+      if (out.num) out.num[ev] = t;
+      if (out.sum) out.sum[ev] = sum;
+      if (out.argmax) out.argmax[ev] = best;
+      if (out.argmask) out.argmask[ev] = t > 0 ? 0 : 1;
+      if (out.max) out.max[ev] = maxv;
+      if (out.maxmask) out.maxmask[ev] = t > 0 ? 0 : 1;
+      continue;
+    }
     // pass 1: number of selected particles
     int64_t m = 0;
     if (staged) {

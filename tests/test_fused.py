@@ -27,7 +27,7 @@ def hip():
 TERMS = [("pt", ">", 25.0), ("eta", "abs<", 2.4)]
 
 
-@pytest.mark.parametrize("sl", [slice(1, None), slice(None), slice(0, 2), slice(-2, None), slice(1, None, 2)])
+@pytest.mark.parametrize("sl", [slice(1, None), slice(None), slice(3, None), slice(0, 2), slice(-2, None), slice(1, None, 2)])
 def test_event_pipeline_matches_oracle(cpu, hip, sl):
     from awkward_amd.fused import EventPipeline
 
