@@ -29,4 +29,3 @@ from awkward_amd.highlevel import (  # noqa: F401
     to_backend,
 )
 from awkward_amd.interop import DLPackDevice, from_buffers, from_dlpack, to_buffers  # noqa: F401
-from awkward_amd.lazy import lazy  # noqa: F401
