@@ -25,6 +25,9 @@
 extern "C" Error awkward_argsort_int64_int64(int64_t* toptr, const int64_t* fromptr, int64_t length,
                                              const int64_t* offsets, int64_t offsetslength, bool ascending,
                                              bool stable);
+extern "C" Error awkward_argsort_int32_int64(int64_t* toptr, const int32_t* fromptr, int64_t length,
+                                             const int64_t* offsets, int64_t offsetslength, bool ascending,
+                                             bool stable);
 
 namespace akhip {
 namespace {
@@ -118,24 +121,120 @@ struct ColMajor {
   int64_t row0, row1;  // rows of the outer bins: [outer_offsets[0], outer_offsets[outlength])
   int64_t E0, n;       // their elements: [offsets[row0], offsets[row1])
   int64_t groups;      // outlength * maxcount
-  int64_t* keys;       // n: column of each element (maxcount: past the last column)
+  // n: column of each element (maxcount: past the last column); 32-bit keys
+  // (the sort's narrow path) whenever maxcount fits
+  int32_t* keys32;
+  int64_t* keys64;
   int64_t* segoff;     // outlength + 1: each bin's first element, relative to E0
   int64_t* order;      // n: argsort of keys within each bin (local positions)
   int64_t* cnt;        // groups: elements per (bin, column)
   int64_t* prefix;     // groups + 1: prefix[g] = elements of the groups before g
   unsigned long long* maxgroup;  // 1 + the last non-empty group (0: none)
+  __device__ int64_t key(int64_t t) const { return keys32 ? (int64_t)keys32[t] : keys64[t]; }
 };
 
-__global__ __launch_bounds__(kBlock) void keys_kernel(ColMajor cm, const int64_t* offsets,
-                                                      const int64_t* outer_offsets, int64_t outlength,
-                                                      int64_t maxcount) {
-  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < cm.n; t += (int64_t)gridDim.x * kBlock) {
-    const int64_t e = cm.E0 + t;
-    const int64_t row = upper_bound(offsets, cm.row0, cm.row1 + 1, e) - 1;
-    const int64_t bin = upper_bound(outer_offsets, 0, outlength + 1, row) - 1;
-    const int64_t col = e - offsets[row];
-    cm.keys[t] = col < maxcount ? col : maxcount;
-    if (col < maxcount) atomicAdd((unsigned long long*)&cm.cnt[bin * maxcount + col], 1ull);
+// Element keys, 2048 consecutive elements per workgroup iteration: the rows
+// they fall in are found once per iteration, each element's row by a
+// search within them.
+constexpr int kKeysTile = kBlock * 8;
+__global__ __launch_bounds__(kBlock) void keys_kernel(ColMajor cm, const int64_t* offsets, int64_t maxcount) {
+  __shared__ int64_t s_rows[2];
+  for (int64_t t0 = (int64_t)blockIdx.x * kKeysTile; t0 < cm.n; t0 += (int64_t)gridDim.x * kKeysTile) {
+    const int64_t t1 = t0 + kKeysTile < cm.n ? t0 + kKeysTile : cm.n;
+    if (threadIdx.x < 2) {
+      const int64_t e = cm.E0 + (threadIdx.x == 0 ? t0 : t1 - 1);
+      s_rows[threadIdx.x] = upper_bound(offsets, cm.row0, cm.row1 + 1, e) - 1;
+    }
+    __syncthreads();
+    const int64_t ra = s_rows[0], rb = s_rows[1];
+    __syncthreads();
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+      const int64_t e = cm.E0 + t;
+      const int64_t row = upper_bound(offsets, ra, rb + 1, e) - 1;
+      const int64_t col = e - offsets[row];
+      const int64_t k = col < maxcount ? col : maxcount;
+      if (cm.keys32) cm.keys32[t] = (int32_t)k;
+      else cm.keys64[t] = k;
+    }
+  }
+}
+
+// cnt[b * maxcount + c] = rows of bin b longer than c.  A workgroup takes
+// 2048 consecutive rows; their bins [b0, b0 + span) share an LDS histogram of
+// row lengths clipped to CL = min(maxcount, kHist / span - 1), turned into
+// "longer than c" counts by a suffix sum and added to cnt once per
+// workgroup -- so rows of one bin do not all hit the same counters.  The
+// columns [CL, length) of rows longer than CL (few, unless the bins are many,
+// when the counters they hit are many too) go to cnt directly, the lanes of
+// a wave splitting each such row.
+constexpr int kHist = 8192;
+constexpr int kCountRows = 2048;
+__global__ __launch_bounds__(kBlock) void colcount_kernel(ColMajor cm, const int64_t* offsets,
+                                                          const int64_t* outer_offsets, int64_t outlength,
+                                                          int64_t maxcount) {
+  __shared__ unsigned int hist[kHist];
+  __shared__ int64_t s_bins[2];
+  __shared__ int s_ml;  // the longest clipped length of the rows (bounds the suffix sums)
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t r0 = cm.row0 + (int64_t)blockIdx.x * kCountRows; r0 < cm.row1;
+       r0 += (int64_t)gridDim.x * kCountRows) {
+    const int64_t r1 = r0 + kCountRows < cm.row1 ? r0 + kCountRows : cm.row1;
+    if (threadIdx.x < 2) s_bins[threadIdx.x] = upper_bound(outer_offsets, 0, outlength + 1,
+                                                           threadIdx.x == 0 ? r0 : r1 - 1) - 1;
+    __syncthreads();
+    const int64_t b0 = s_bins[0], span = s_bins[1] - s_bins[0] + 1;
+    __syncthreads();
+    int64_t cl = kHist / span - 1;
+    if (cl > maxcount) cl = maxcount;
+    if (cl < 0) cl = 0;
+    const int64_t w = cl + 1;  // histogram entries per bin: clipped lengths 0 .. cl
+    if (cl > 0)
+      for (int64_t i = threadIdx.x; i < span * w; i += kBlock) hist[i] = 0u;
+    if (threadIdx.x == 0) s_ml = 0;
+    __syncthreads();
+    for (int64_t rbase = r0; rbase < r1; rbase += kBlock) {
+      const int64_t r = rbase + threadIdx.x;
+      const bool valid = r < r1;
+      int64_t b = 0, len = 0;
+      if (valid) {
+        b = upper_bound(outer_offsets, b0, b0 + span + 1 <= outlength + 1 ? b0 + span + 1 : outlength + 1, r) - 1;
+        len = offsets[r + 1] - offsets[r];
+        len = len < 0 ? 0 : len > maxcount ? maxcount : len;
+        if (cl > 0) {
+          const int64_t k = len < cl ? len : cl;
+          atomicAdd(&hist[(b - b0) * w + k], 1u);
+          atomicMax(&s_ml, (int)k);
+        }
+      }
+      uint64_t over = __ballot(valid && len > cl);
+      while (over) {
+        const int src = __builtin_ctzll(over);
+        over &= over - 1;
+        const int64_t ob = __shfl(b, src, kWave), olen = __shfl(len, src, kWave);
+        for (int64_t c = cl + lane; c < olen; c += kWave)
+          atomicAdd((unsigned long long*)&cm.cnt[ob * maxcount + c], 1ull);
+      }
+    }
+    __syncthreads();
+    const int64_t ml = s_ml;
+    if (ml > 0) {
+      // per bin: entry c + 1 becomes the rows of length > c (lengths clipped at
+      // cl; entries above ml are zero)
+      for (int64_t bb = threadIdx.x; bb < span; bb += kBlock) {
+        unsigned int run = 0;
+        for (int64_t c = ml - 1; c >= 0; c--) {
+          run += hist[bb * w + c + 1];
+          hist[bb * w + c + 1] = run;
+        }
+      }
+      __syncthreads();
+      for (int64_t i = threadIdx.x; i < span * ml; i += kBlock) {
+        const int64_t bb = i / ml, c = i - bb * ml;
+        const unsigned int v = hist[bb * w + c + 1];
+        if (v) atomicAdd((unsigned long long*)&cm.cnt[(b0 + bb) * maxcount + c], (unsigned long long)v);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -168,24 +267,36 @@ Error colmajor(ColMajor& cm, const int64_t* offsets, const int64_t* outer_offset
   cm.n = read_device(offsets, cm.row1) - cm.E0;
   if (cm.n < 0) cm.n = 0;
   cm.groups = outlength * maxcount;
-  const size_t nb = round256((size_t)cm.n * 8), sb = round256((size_t)(outlength + 1) * 8),
-               gb = round256((size_t)cm.groups * 8), pb = round256((size_t)(cm.groups + 1) * 8);
-  char* q = (char*)queue_pool(256 + 2 * nb + sb + gb + pb);
+  const bool narrow = maxcount < INT32_MAX;
+  const size_t nb = round256((size_t)cm.n * 8), kb = round256((size_t)cm.n * (narrow ? 4 : 8)),
+               sb = round256((size_t)(outlength + 1) * 8), gb = round256((size_t)cm.groups * 8),
+               pb = round256((size_t)(cm.groups + 1) * 8);
+  char* q = (char*)queue_pool(256 + nb + kb + sb + gb + pb);
   if (!q) return failure("out of device memory for the column regrouping", kSliceNone, kSliceNone, kFile);
   cm.maxgroup = (unsigned long long*)q;
-  cm.keys = (int64_t*)(q + 256);
-  cm.order = (int64_t*)(q + 256 + nb);
-  cm.segoff = (int64_t*)(q + 256 + 2 * nb);
-  cm.cnt = (int64_t*)(q + 256 + 2 * nb + sb);
-  cm.prefix = (int64_t*)(q + 256 + 2 * nb + sb + gb);
+  cm.keys32 = narrow ? (int32_t*)(q + 256 + nb + sb + gb + pb) : nullptr;
+  cm.keys64 = narrow ? nullptr : (int64_t*)(q + 256 + nb + sb + gb + pb);
+  cm.order = (int64_t*)(q + 256);
+  cm.segoff = (int64_t*)(q + 256 + nb);
+  cm.cnt = (int64_t*)(q + 256 + nb + sb);
+  cm.prefix = (int64_t*)(q + 256 + nb + sb + gb);
   set_i64<<<1, 1, 0, st>>>((int64_t*)cm.maxgroup, 0);
   if (cm.groups > 0) fill_kernel<<<grid_for(cm.groups, kBlock, 8), kBlock, 0, st>>>(cm.cnt, cm.groups, 0);
   segoff_kernel<<<(unsigned)ceil_div(outlength + 1, (int64_t)kBlock), kBlock, 0, st>>>(cm, offsets, outer_offsets,
                                                                                        outlength);
+  if (cm.row1 > cm.row0 && cm.groups > 0) {
+    int64_t grid = ceil_div(cm.row1 - cm.row0, (int64_t)kCountRows);
+    if (grid > (int64_t)cu_count() * 4) grid = (int64_t)cu_count() * 4;
+    colcount_kernel<<<(unsigned)grid, kBlock, 0, st>>>(cm, offsets, outer_offsets, outlength, maxcount);
+    AKHIP_LAUNCH_CHECK("colcount_kernel");
+  }
   if (cm.n > 0) {
-    keys_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, offsets, outer_offsets, outlength, maxcount);
+    int64_t grid = ceil_div(cm.n, (int64_t)kKeysTile);
+    if (grid > (int64_t)cu_count() * 16) grid = (int64_t)cu_count() * 16;
+    keys_kernel<<<(unsigned)grid, kBlock, 0, st>>>(cm, offsets, maxcount);
     AKHIP_LAUNCH_CHECK("keys_kernel");
-    Error e = awkward_argsort_int64_int64(cm.order, cm.keys, cm.n, cm.segoff, outlength + 1, true, true);
+    Error e = cm.keys32 ? awkward_argsort_int32_int64(cm.order, cm.keys32, cm.n, cm.segoff, outlength + 1, true, true)
+                        : awkward_argsort_int64_int64(cm.order, cm.keys64, cm.n, cm.segoff, outlength + 1, true, true);
     if (e.str) return e;
   }
   set_i64<<<1, 1, 0, st>>>(cm.prefix, 0);
@@ -208,7 +319,7 @@ __device__ inline Sorted sorted_at(const ColMajor& cm, int64_t t, int64_t outlen
   const int64_t b = upper_bound(cm.segoff, 0, outlength + 1, t) - 1;
   const int64_t local = cm.order[t];
   const int64_t rel = cm.segoff[b] + local;
-  return Sorted{b, cm.E0 + rel, cm.keys[rel], t - cm.segoff[b]};
+  return Sorted{b, cm.E0 + rel, cm.key(rel), t - cm.segoff[b]};
 }
 
 __global__ __launch_bounds__(kBlock) void place_kernel(ColMajor cm, int64_t* nextcarry, int64_t outlength,
@@ -251,13 +362,24 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(int64_t* out, const int6
     out[k] = src[index[k]];
 }
 
-// nummissing[k] after the CPU loop: rows of the last bin shorter than k + 1
+// nummissing[k] after the CPU loop: rows of the last bin shorter than k + 1.
+// A histogram of the short rows' lengths, per workgroup in LDS for lengths
+// below kHist (the common ones), added to `hist` once per workgroup.
 __global__ __launch_bounds__(kBlock) void short_rows_kernel(int64_t* hist, const int64_t* offsets, int64_t r0,
                                                             int64_t r1, int64_t maxcount) {
+  __shared__ unsigned int h[kHist];
+  const int64_t lim = maxcount < kHist ? maxcount : kHist;
+  for (int64_t i = threadIdx.x; i < lim; i += kBlock) h[i] = 0u;
+  __syncthreads();
   for (int64_t r = r0 + (int64_t)blockIdx.x * kBlock + threadIdx.x; r < r1; r += (int64_t)gridDim.x * kBlock) {
-    const int64_t c = offsets[r + 1] - offsets[r];
-    if (c < maxcount) atomicAdd((unsigned long long*)&hist[c > 0 ? c : 0], 1ull);
+    int64_t c = offsets[r + 1] - offsets[r];
+    c = c > 0 ? c : 0;
+    if (c < lim) atomicAdd(&h[c], 1u);
+    else if (c < maxcount) atomicAdd((unsigned long long*)&hist[c], 1ull);
   }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < lim; i += kBlock)
+    if (h[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)h[i]);
 }
 __global__ void nummissing_kernel(int64_t* nummissing, const int64_t* hist, int64_t maxcount) {
   int64_t run = 0;

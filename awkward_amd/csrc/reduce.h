@@ -34,6 +34,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "errors.h"
 
 namespace akhip {
 
@@ -527,12 +528,19 @@ namespace akhip {
 // global index becomes local (awkward_NumpyArray_reduce_adjust_starts_64 with
 // starts = offsets[:-1]) and the empty-list mask is written in the same pass
 // (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64).
+//
+// Below an option type the reduced content was compacted: `shifts[i]` counts
+// the nulls removed before element i and `starts[bin]` is the list's start
+// before compaction, so the local index is v + shifts[v] - starts[bin]
+// (awkward_NumpyArray_reduce_adjust_starts_shifts_64).
 struct LocalArgEpilogue {
   int64_t* toptr;
-  int8_t* mask;  // may be null
+  int8_t* mask;                     // may be null
+  const int64_t* shifts = nullptr;  // null: starts are the offsets
+  const int64_t* starts = nullptr;
   __device__ void store(int64_t bin, int64_t v, int64_t gb, int64_t ge) const {
     if (mask) mask[bin] = (ge - gb > 0) ? 0 : 1;
-    toptr[bin] = v >= 0 ? v - gb : v;
+    toptr[bin] = v < 0 ? v : shifts ? v + shifts[v] - starts[bin] : v - gb;
   }
 };
 
@@ -557,6 +565,54 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, const typename Op::In
   return success();
 }
 
+// ---- long lists across the chip ---------------------------------------------------------
+// Few lists of many elements (a reduction across lists regroups the data
+// into maxcount columns): a tile pass that queues every list longer than
+// kWaveLen in kReduceChunk-element chunks, one chunk per workgroup
+// iteration over the whole chip, and the chunk partials combined in order
+// per list.  The queue is sized from the element count, read from the
+// offsets (one host synchronisation), so no list stays with a workgroup.
+template <typename In, class Epi>
+struct SingleSrc {
+  const In* p;
+  Epi e;
+  __device__ const In* data(int) const { return p; }
+  __device__ Epi epi(int) const { return e; }
+};
+
+template <class Op, class Epi>
+__global__ __launch_bounds__(256) void segmented_reduce_queued_kernel(Op op, Epi epi,
+                                                                      const typename Op::In* __restrict__ fromptr,
+                                                                      const int64_t* __restrict__ offsets,
+                                                                      int64_t outlength, ReduceQueue q) {
+  __shared__ ReduceSmem<Op, 256, 1, 8192> sm;
+  reduce_tile<Op, 256, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength, (int64_t)blockIdx.x * 256, sm, q, 0);
+}
+
+template <class Op, class Epi>
+inline Error launch_reduce_queued(const Op& op, const Epi& epi, const typename Op::In* fromptr,
+                                  const int64_t* offsets, int64_t outlength) {
+  using Part = typename Op::Part;
+  using In = typename Op::In;
+  if (outlength <= 0) return success();
+  const int64_t elements = read_device(offsets, outlength) - read_device(offsets, 0);
+  const int64_t cap = reduce_queue_capacity(elements > 0 ? elements : 1);
+  const size_t cb = ((size_t)cap * sizeof(ReduceChunk) + 255) / 256 * 256;
+  const size_t pb = ((size_t)cap * sizeof(Part) + 255) / 256 * 256;
+  char* m = (char*)scratch(256 + cb + pb);
+  if (!m) return failure("out of device memory for the long-list queue", kSliceNone, kSliceNone, "awkward_amd/csrc/reduce.h");
+  const ReduceQueue q{(ReduceChunk*)(m + 256), (void*)(m + 256 + cb), (unsigned long long*)m, cap};
+  hipStream_t st = stream();
+  reduce_queue_reset<><<<1, 1, 0, st>>>(q.count);
+  segmented_reduce_queued_kernel<Op, Epi><<<(unsigned)ceil_div(outlength, (int64_t)256), 256, 0, st>>>(
+      op, epi, fromptr, offsets, outlength, q);
+  const SingleSrc<In, Epi> src{fromptr, epi};
+  reduce_chunks_kernel<Op, SingleSrc<In, Epi>><<<(unsigned)cu_count() * 8, 256, 0, st>>>(op, src, q);
+  reduce_finalize_kernel<Op, SingleSrc<In, Epi>><<<(unsigned)cu_count(), 256, 0, st>>>(op, src, q);
+  AKHIP_LAUNCH_CHECK("segmented_reduce_queued_kernel");
+  return success();
+}
+
 // Tile variants (threads, lists per thread, LDS staging bytes).  The host
 // layer picks one from the average list length
 // (elements / lists, known from the array shapes without a device sync), and
@@ -566,7 +622,10 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, const typename Op::In
 //   3: 128 x 1, 16 KiB    4: 256 x 1, 64 KiB    5: 256 x 8, 8 KiB
 //   6:  64 x 1,  8 KiB    7: 128 x 2, 16 KiB    8:  64 x 4,  8 KiB
 //   9: 128 x 1,  8 KiB
+//  10: the long-list queue above (chosen for mean lengths of thousands; not
+//      one of the searched tiles: it synchronises once)
 constexpr int kVariants = 10;
+constexpr int kLongLists = 10;
 
 // The plain entry points use variant 3 (128 threads, one list each, 16 KiB):
 // the fastest tile for lists of ~4-30 elements at every size measured.
@@ -581,6 +640,7 @@ template <class Op, class Epi>
 inline Error launch_reduce(const Op& op, const typename Op::In* fromptr, const int64_t* offsets, int64_t outlength,
                            int variant, Epi epi) {
   switch (variant) {
+    case kLongLists: return launch_reduce_queued<Op, Epi>(op, epi, fromptr, offsets, outlength);
     case 1: return launch_reduce_v<Op, 256, 4, 8192, Epi>(op, epi, fromptr, offsets, outlength);
     case 2: return launch_reduce_v<Op, 256, 2, 16384, Epi>(op, epi, fromptr, offsets, outlength);
     case 3: return launch_reduce_v<Op, 128, 1, 16384, Epi>(op, epi, fromptr, offsets, outlength);

@@ -1,7 +1,9 @@
 // reduce_argmax.hip -- awkward_reduce_argmax_<T>_64 (awkward-cpp/src/cpu-kernels/awkward_reduce_argmax.cpp),
 // plus awkward_hip_reduce_argmax_local_<T>: the same reduction with an explicit tile variant whose
 // epilogue also makes the index local (awkward_NumpyArray_reduce_adjust_starts_64 with
-// starts = offsets[:-1]) and writes the empty-list mask; no counterpart in awkward-cpp.
+// starts = offsets[:-1]) and writes the empty-list mask, and awkward_hip_reduce_argmax_shifts_<T>:
+// the same below an option type (awkward_NumpyArray_reduce_adjust_starts_shifts_64 fused); no
+// counterpart in awkward-cpp.
 
 #include "reduce.h"
 
@@ -18,6 +20,12 @@ extern "C" {
                                           const int64_t* offsets, int64_t outlength, int variant) {        \
     return launch_reduce(ArgOp<T, true>{}, fromptr, offsets, outlength, variant, \
                          LocalArgEpilogue{toptr, mask});                                                          \
+  }                                                                                                        \
+  Error awkward_hip_reduce_argmax_shifts_##TN(int64_t* toptr, int8_t* mask, const T* fromptr,                \
+                                           const int64_t* offsets, const int64_t* starts,                    \
+                                           const int64_t* shifts, int64_t outlength, int variant) {          \
+    return launch_reduce(ArgOp<T, true>{}, fromptr, offsets, outlength, variant,                              \
+                         LocalArgEpilogue{toptr, mask, shifts, starts});                                             \
   }
 ARG(int8, int8_t)
 ARG(uint8, uint8_t)

@@ -67,7 +67,16 @@ def heuristic_variant(n_lists: int, n_elements: int, itemsize: int) -> int:
     return 4 if best is None else best  # long lists: the largest stage, heavy paths
 
 
+#: the long-list queue (``csrc/reduce.h``, ``kLongLists``): chunks of very long
+#: lists spread over the chip instead of one workgroup per list
+LONG_LISTS = 10
+#: mean list length from which the long-list queue is used
+LONG_MEAN = 2048
+
+
 def choose_variant(op: str, n_lists: int, n_elements: int, itemsize: int, offsets=None) -> int:
+    if n_lists > 0 and n_elements >= LONG_MEAN * n_lists:
+        return LONG_LISTS
     if chooser is not None:
         return int(chooser(op, n_lists, n_elements, itemsize, offsets))
     return heuristic_variant(n_lists, n_elements, itemsize)
@@ -119,14 +128,21 @@ def reduce_extremum(kind, out, mask, data, offsets, n, identity, variant):
              device_address(offsets), n, identity, variant)
 
 
-def reduce_arg(kind, out, mask, data, offsets, n, variant):
+def reduce_arg(kind, out, mask, data, offsets, n, variant, shifts=None, starts=None):
+    """Local argmin/argmax; with ``shifts`` (below an option type) the index
+    is made local to ``starts`` (the lists' starts before the nulls were
+    compacted away) instead of to ``offsets``."""
     from awkward_amd._kernels import device_address, set_stream
 
     tn = np.dtype(_np(data.dtype)).name
-    f = _fn(f"awkward_hip_reduce_{kind}_local_{tn}", [P, P, P, P, I64, INT])
     set_stream()
-    return f(device_address(out), 0 if mask is None else device_address(mask), device_address(data),
-             device_address(offsets), n, variant)
+    m = 0 if mask is None else device_address(mask)
+    if shifts is not None:
+        f = _fn(f"awkward_hip_reduce_{kind}_shifts_{tn}", [P, P, P, P, P, P, I64, INT])
+        return f(device_address(out), m, device_address(data), device_address(offsets), device_address(starts),
+                 device_address(shifts), n, variant)
+    f = _fn(f"awkward_hip_reduce_{kind}_local_{tn}", [P, P, P, P, I64, INT])
+    return f(device_address(out), m, device_address(data), device_address(offsets), n, variant)
 
 
 def _np(dtype):

@@ -135,12 +135,13 @@ class _Arg(Reducer):
             data = nplike.view(data, np.int8)
             dtype = np.dtype(np.int8)
         result = nplike.empty(outlength, np.int64)
-        if _fused(array) and shifts is None:
+        if _fused(array):
             from awkward_amd import dispatch
 
             outmask = nplike.empty(outlength, np.int8) if mask else None
             array._check(dispatch.reduce_arg(self.name, result, outmask, data, offsets, outlength,
-                                             _variant(self.name, array, outlength, offsets)))
+                                             _variant(self.name, array, outlength, offsets),
+                                             shifts, starts))
             return type(array)(result, array.backend), outmask
         array._check(
             array.backend[self.kernel, np.int64, dtype, np.int64, np.int64](result, data, offsets, starts, outlength)

@@ -397,6 +397,22 @@ def roofline_probe(args):
     dev = [(torch.from_numpy(o).cuda(), torch.from_numpy(v).cuda()) for o, v in batch]
     BatchRunner(dev, variants={n: 0 for n in ("int32", "int64", "float32", "float64")}).run()
     torch.cuda.synchronize()
+    del dev
+    # option types and non-local reductions on the config-1 layout: 10% of the
+    # values missing (IndexedOptionArray), and the same lists reduced across
+    # (axis=0: by position)
+    from awkward_amd import contents as C
+
+    index = np.arange(len(x), dtype=np.int64)
+    index[rng.random(len(x)) < 0.1] = -1
+    opt = ak.Array(C.ListOffsetArray(tuned.nplike.asarray(off), C.IndexedOptionArray(
+        tuned.nplike.asarray(index), C.NumpyArray(tuned.nplike.asarray(x), tuned), tuned), tuned))
+    for fn in (ak.sum, ak.argmax):
+        fn(opt, axis=-1)
+    arr = ak.from_offsets(off, x, tuned)
+    for fn in (ak.sum, ak.argmax):
+        fn(arr, axis=0)
+    torch.cuda.synchronize()
     print("roofline probe done", flush=True)
 
 

@@ -28,7 +28,9 @@ int awkward_hip_reserve_queue(int64_t bytes);
 /* Number of reduce tile variants (threads x lists per thread x LDS stage). */
 int awkward_hip_reduce_variant_count(void);
 
-/* ---- tuned reductions: one tile variant (0 .. 9), fused epilogues ---------
+/* ---- tuned reductions: one tile variant (0 .. 9, or 10: lists longer than
+   4096 elements queued in 8192-element chunks across the chip, which reads the
+   element count from the offsets), fused epilogues ---------
    sum:  awkward_reduce_sum_<OUT>_<IN>_64 with the variant's tile.
    max/min_masked: awkward_reduce_max/min + the empty-list mask
      (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64) in one pass; mask may
@@ -102,6 +104,32 @@ struct Error awkward_hip_reduce_argmin_local_int64(int64_t* toptr, int8_t* mask,
 struct Error awkward_hip_reduce_argmin_local_uint64(int64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_argmin_local_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_argmin_local_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+/* arg-reductions below an option type: the content was compacted, shifts[i]
+   counts the nulls removed before element i and starts[k] is list k's start
+   before compaction; the local index is v + shifts[v] - starts[k]
+   (awkward_NumpyArray_reduce_adjust_starts_shifts_64 fused, replacing the
+   sequence at awkward/_reducers.py:85-129 with shifts; reference
+   awkward-cpp/src/cpu-kernels/awkward_NumpyArray_reduce_adjust_starts_shifts_64.cpp). */
+struct Error awkward_hip_reduce_argmax_shifts_int8(int64_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_uint8(int64_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_int16(int64_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_uint16(int64_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_int32(int64_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_uint32(int64_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_uint64(int64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmax_shifts_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_int8(int64_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_uint8(int64_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_int16(int64_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_uint16(int64_t* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_int32(int64_t* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_uint32(int64_t* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_int64(int64_t* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_uint64(int64_t* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_argmin_shifts_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
 
 /* ---- gathers and small host-layer steps ------------------------------------
    take_<bytes>: out[i] = data[index[i]] (NumpyArray._carry, numpyarray.py:367-373).

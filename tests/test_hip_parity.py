@@ -443,3 +443,64 @@ def test_tuned_variants(variant, t):
         d_out = torch.empty(n, dtype=torch.int64, device="cuda")
         assert dispatch.reduce_arg("argmin", d_out, d_mask, d_x, d_off, n, variant).str is None
         assert same_bits(want, d_out.cpu().numpy()) and same_bits(hm[0], d_mask.cpu().numpy()), ("argmin", label)
+
+
+@pytest.mark.parametrize("t", ["int32", "int64", "float32", "float64"])
+def test_long_list_queue_and_shifts(t):
+    """Variant 10 (lists longer than 4096 elements split into 8192-element
+    chunks across the chip, partials combined in order) on a few very long
+    lists among short and empty ones, and the arg-reductions' shifts entry
+    points (adjust_starts_shifts in the epilogue), against the oracle."""
+    import torch
+
+    from awkward_amd import dispatch
+
+    rng = np.random.default_rng(77)
+    counts = np.array([300000, 0, 5, 4097, 200000, 1, 0, 8192, 8193, 123457, 4096, 2, 40000], dtype=np.int64)
+    off = offsets_from_counts(counts)
+    n = len(counts)
+    x = values(rng, t, int(off[-1]), special=False)
+    if t.startswith("int"):
+        x[rng.integers(0, len(x), 64)] = np.iinfo(t).max  # ties of the maximum across chunks
+    d_off = torch.from_numpy(off).cuda()
+    d_x = torch.from_numpy(x).cuda()
+    out_t = {"int32": "int64", "int64": "int64", "float32": "float32", "float64": "float64"}[t]
+    h, _, _, _ = run_both(f"awkward_reduce_sum_{out_t}_{t}_64", [np.zeros(n, out_t), x, off, n])
+    d_out = torch.empty(n, dtype=getattr(torch, out_t), device="cuda")
+    assert dispatch.reduce_sum(d_out, d_x, d_off, n, dispatch.LONG_LISTS).str is None
+    got = d_out.cpu().numpy()
+    if out_t.startswith("int"):
+        assert same_bits(h[0], got)
+    else:
+        eps = np.finfo(out_t).eps
+        absum = np.add.reduceat(np.abs(x.astype(np.float64)), off[:-1]) if len(x) else np.zeros(n)
+        absum[counts == 0] = 0
+        bound = 2 * counts * eps * absum
+        assert np.all(np.abs(got.astype(np.float64) - h[0].astype(np.float64)) <= bound)
+    hm, _, _, _ = run_both("awkward_NumpyArray_reduce_mask_ByteMaskedArray_64", [np.zeros(n, np.int8), off, n])
+    d_mask = torch.empty(n, dtype=torch.int8, device="cuda")
+    for kind in ("max", "min"):
+        ident = (-np.inf if kind == "max" else np.inf) if t.startswith("float") else \
+            int(np.iinfo(t).min if kind == "max" else np.iinfo(t).max)
+        h, _, _, _ = run_both(f"awkward_reduce_{kind}_{t}_{t}_64", [np.zeros(n, t), x, off, n, ident])
+        d_val = torch.empty(n, dtype=getattr(torch, t), device="cuda")
+        assert dispatch.reduce_extremum(kind, d_val, d_mask, d_x, d_off, n, ident, dispatch.LONG_LISTS).str is None
+        assert same_bits(h[0], d_val.cpu().numpy()) and same_bits(hm[0], d_mask.cpu().numpy()), kind
+    shifts = np.cumsum(rng.integers(0, 3, len(x))).astype(np.int64)
+    starts = off[:-1] + np.cumsum(rng.integers(0, 5, n))  # the lists' starts before compaction
+    d_shifts = torch.from_numpy(shifts).cuda()
+    d_starts = torch.from_numpy(starts).cuda()
+    for kind in ("argmax", "argmin"):
+        h, _, _, _ = run_both(f"awkward_reduce_{kind}_{t}_64", [np.zeros(n, np.int64), x, off, off[:-1].copy(), n])
+        local = np.where(h[0] >= 0, h[0] - off[:-1], h[0])
+        hs, _, _, _ = run_both("awkward_NumpyArray_reduce_adjust_starts_shifts_64",
+                               [h[0].copy(), n, np.zeros(len(x), np.int64), starts, shifts])
+        for variant in (dispatch.LONG_LISTS, 3):
+            d_out = torch.empty(n, dtype=torch.int64, device="cuda")
+            assert dispatch.reduce_arg(kind, d_out, d_mask, d_x, d_off, n, variant).str is None
+            assert same_bits(local, d_out.cpu().numpy()), (kind, variant)
+            assert same_bits(hm[0], d_mask.cpu().numpy()), (kind, variant)
+            assert dispatch.reduce_arg(kind, d_out, None, d_x, d_off, n, variant, shifts=d_shifts,
+                                       starts=d_starts).str is None
+            assert same_bits(hs[0], d_out.cpu().numpy()), (kind, variant, "shifts")
+

@@ -195,3 +195,32 @@ def test_nonlocal_reducers_hip_vs_oracle():
         out[label] = {(op, axis): getattr(ak, op)(a, axis=axis).to_list()
                       for op in ("sum", "max", "min", "argmax", "argmin", "count") for axis in (0, 1, 2)}
     assert out["hip"] == out["oracle"]
+
+
+@pytest.mark.gpu
+def test_axis0_long_columns_hip_vs_oracle():
+    """axis=0 on 200,000 lists of Poisson(10) lengths: a few columns of up to
+    200,000 elements each, which the reducers hand to the long-list queue;
+    with 10% of the values missing (IndexedOptionArray) the arg-reductions
+    also take the shifts entry points."""
+    from awkward_amd import contents as C
+
+    rng = np.random.default_rng(12)
+    counts = rng.poisson(10, 200_000)
+    off = offsets_from_counts(counts)
+    x = rng.integers(-50, 50, int(off[-1])).astype(np.int64)
+    index = np.arange(len(x), dtype=np.int64)
+    index[rng.random(len(x)) < 0.1] = -1
+    out = {}
+    for label, make in (("oracle", oracle_backend), ("hip", hip_backend)):
+        be = make()
+        nl = be.nplike
+        plain = ak.from_offsets(off, x, be)
+        opt = ak.Array(C.ListOffsetArray(nl.asarray(off), C.IndexedOptionArray(
+            nl.asarray(index), C.NumpyArray(nl.asarray(x), be), be), be))
+        out[label] = {(op, name): getattr(ak, op)(arr, axis=0).to_list()
+                      for op in ("sum", "max", "argmax", "argmin", "count")
+                      for name, arr in (("plain", plain), ("option", opt))}
+    for key in out["oracle"]:
+        assert out["hip"][key] == out["oracle"][key], key
+
