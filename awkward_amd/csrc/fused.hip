@@ -12,12 +12,15 @@
 // ListArray_getitem_jagged_carrylen/_apply, ListArray_getitem_next_range
 // _carrylength/_next_range, two gathers, awkward_reduce_sum and
 // awkward_reduce_argmax + adjust_starts + reduce_mask.  Here a workgroup of
-// 256 events stages the fields of its particles in LDS (one coalesced pass),
-// then each thread walks its event twice: once to count the selected
-// particles (the slice is regularised against that count, as
-// awkward_regularize_rangeslice does) and once to reduce the ones the slice
-// keeps, in particle order -- the same serial order as the per-kernel path,
-// so sums are bit-identical.
+// events loads the float32 fields of its particles in one coalesced pass,
+// evaluates the predicate on the registers they land in and keeps only the
+// reduced field and a flag per particle in LDS; then each thread walks its
+// event: once to count the selected particles (the slice is regularised
+// against that count, as awkward_regularize_rangeslice does) and once to
+// reduce the ones the slice keeps, in particle order -- the same serial
+// order as the per-kernel path, so sums are bit-identical.  A [start:] slice
+// with step 1 needs no count (the kept particles are the selected ones of
+// rank >= start): one walk.
 //
 // awkward_hip_pair_max: for every event, the maximum and first argmax over
 // its particle pairs (i < j, lexicographic, as awkward_ListArray_combinations
@@ -120,33 +123,30 @@ struct PipelineOut {
   int8_t* maxmask;
 };
 
-// Staging plan: slot[k] is field k's row in the LDS stage (-1: read from
-// global memory); the stage holds nslots rows of cap floats.
+// Staging plan: slot[k] >= 0 for the float32 fields loaded by the staging
+// pass (-1: read from global memory where a term needs it); the LDS stage
+// holds cap floats of the reduced field and cap flag bytes.
 struct Stage {
   int slot[kMaxFields];
   int nslots;
   int cap;
 };
 
-// A term resolved once per thread before the particle loops: its LDS row
-// (or global column), type, operator and constant live in registers, so the
-// loops never re-read kernel arguments.
+// A term resolved once per thread before the particle loops: its field
+// (loaded by the staging pass, or a global column), type, operator and
+// constant live in registers, so the loops never re-read kernel arguments.
 struct TermRT {
-  const float* lds;  // staged float32 row, or null
   const void* gptr;  // global column
   int type;
   int op;
   int absolute;
   float fval;
   double dval;
+  int field;
+  bool staged;  // the field is loaded into registers by the staging loop
 };
 
 __device__ inline bool eval_rt(const TermRT& t, int64_t i, int64_t local) {
-  if (t.lds) {
-    float x = t.lds[local];
-    if (t.absolute) x = fabsf(x);
-    return compare<float>(x, t.op, t.fval);
-  }
   // float32 compares in float32 against the constant rounded to float32, as
   // numpy's weak-scalar rule and torch's scalar comparisons do
   if (t.type == F32) {
@@ -170,9 +170,10 @@ __device__ inline bool keep_particle(const TermRT* rt, int64_t i, int64_t local)
 }
 
 // BLOCK threads, LPT events each (thread t owns events t + j * BLOCK, so
-// output stores coalesce); the workgroup's particles are staged with
-// coalesced loads -- every staged field's loads issued together -- into a
-// stage sized at launch (dynamic LDS), to keep many loads in flight per CU.
+// output stores coalesce); the workgroup's particles are loaded with
+// coalesced loads -- every float32 field's loads issued together -- and
+// staged (reduced field + predicate flags) in an LDS stage sized at launch
+// (dynamic LDS), to keep many loads in flight per CU.
 template <int BLOCK, int LPT, int NT>
 __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
                                                                Fields fields, Predicate pred, Stage st,
@@ -202,14 +203,31 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
   const bool staged = span <= st.cap;
+  TermRT rt[NT > 0 ? NT : 1];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const Term& term = pred.term[t];
+    rt[t].gptr = fields.ptr[term.field];
+    rt[t].type = fields.type[term.field];
+    rt[t].op = term.op;
+    rt[t].absolute = term.absolute;
+    rt[t].fval = (float)term.value;
+    rt[t].dval = term.value;
+    rt[t].field = term.field;
+    rt[t].staged = st.slot[term.field] >= 0;
+  }
+  // staged: the float32 fields' particles are loaded with coalesced loads;
+  // the predicate is evaluated on the registers they land in (terms on other
+  // fields read global memory), and only the reduced field and one flag byte
+  // per particle go to LDS
+  float* red_row = s_stage;
+  uint8_t* s_flag = (uint8_t*)(s_stage + st.cap);
   if (staged) {
     const float* src[kMaxFields];
-    float* dst[kMaxFields];
 #pragma unroll
     for (int k = 0; k < kMaxFields; k++) {
       const int slot = k < fields.nfields ? st.slot[k] : -1;
       src[k] = slot >= 0 ? (const float*)fields.ptr[k] + E0 : nullptr;
-      dst[k] = slot >= 0 ? s_stage + slot * st.cap : nullptr;
     }
     const int sp = (int)span;
     for (int b = 0; b < sp; b += BLOCK * U) {
@@ -219,41 +237,39 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int q = b + u * BLOCK + tid;
-          if (src[k] && q < sp) r[k][u] = src[k][q];
+          r[k][u] = (src[k] && q < sp) ? src[k][q] : 0.0f;
         }
 #pragma unroll
-      for (int k = 0; k < kMaxFields; k++)
+      for (int u = 0; u < U; u++) {
+        const int q = b + u * BLOCK + tid;
+        if (q < sp) {
+          bool keep = true;
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int q = b + u * BLOCK + tid;
-          if (dst[k] && q < sp) dst[k][q] = r[k][u];
+          for (int t = 0; t < NT; t++) {
+            if (!keep) break;
+            if (rt[t].staged) {
+              float x = 0.0f;
+#pragma unroll
+              for (int k = 0; k < kMaxFields; k++)
+                if (k == rt[t].field) x = r[k][u];
+              if (rt[t].absolute) x = fabsf(x);
+              keep = compare<float>(x, rt[t].op, rt[t].fval);
+            } else {
+              keep = eval_rt(rt[t], E0 + q, q);
+            }
+          }
+          s_flag[q] = keep ? 1 : 0;
+          float v = 0.0f;
+#pragma unroll
+          for (int k = 0; k < kMaxFields; k++)
+            if (k == red) v = r[k][u];
+          red_row[q] = v;
         }
+      }
     }
   }
   __syncthreads();
-  TermRT rt[NT > 0 ? NT : 1];
-#pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const Term& term = pred.term[t];
-    const int slot = st.slot[term.field];
-    rt[t].lds = (staged && slot >= 0) ? s_stage + slot * st.cap : nullptr;
-    rt[t].gptr = fields.ptr[term.field];
-    rt[t].type = fields.type[term.field];
-    rt[t].op = term.op;
-    rt[t].absolute = term.absolute;
-    rt[t].fval = (float)term.value;
-    rt[t].dval = term.value;
-  }
-  // the predicate, once per particle, element-parallel (no divergence):
-  // one flag byte per staged particle, after the float rows (a workgroup
-  // scan of the flags into ranks measured slower than the per-event count)
-  uint8_t* s_flag = (uint8_t*)(s_stage + st.nslots * st.cap);
-  if (staged) {
-    for (int q = tid; q < (int)span; q += BLOCK) s_flag[q] = keep_particle<NT>(rt, E0 + q, q) ? 1 : 0;
-  }
-  __syncthreads();
-  const int red_slot = st.slot[red];
-  const float* red_lds = (staged && red_slot >= 0) ? s_stage + red_slot * st.cap : nullptr;
+  const float* red_lds = staged ? red_row : nullptr;
   const float* red_g = (const float*)fields.ptr[red];
   // [start:] with step 1 (start >= 0, or omitted): the kept particles are the
   // selected ones of rank >= start, known without the count -- one pass
@@ -546,7 +562,7 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_pa
   cap = (cap + 63) / 64 * 64;
   if (cap > 12288) cap = 12288;
   st.cap = (int)cap;
-  const size_t lds = (size_t)st.nslots * st.cap * sizeof(float) + (size_t)st.cap;  // + predicate flags
+  const size_t lds = (size_t)st.cap * sizeof(float) + (size_t)st.cap;  // the reduced field + predicate flags
   const unsigned grid = (unsigned)ceil_div(n, (int64_t)B * lpt);
   hipStream_t s = stream();
   switch (lpt * 16 + nterms) {
