@@ -222,17 +222,24 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
         else hi = i;
       }
       int xa = lo, yb = d - lo;
+      int pa = pad16(A0 + (xa < r ? xa : r - 1));
+      int pb = pad16(B0 + (yb < r ? yb : r - 1));
+      K ka = s_key[pa], kb = s_key[pb];
+      uint32_t ia = s_idx[pa], ib = s_idx[pb];
 #pragma unroll
       for (int u = 0; u < PER; u++) {
-        const int pa = pad16(A0 + (xa < r ? xa : r - 1));
-        const int pb = pad16(B0 + (yb < r ? yb : r - 1));
-        const K ka = s_key[pa], kb = s_key[pb];
-        const uint32_t ia = s_idx[pa], ib = s_idx[pb];
         const bool from_a = yb >= r || (xa < r && !pair_less(kb, ib, ka, ia));
         k[u] = from_a ? ka : kb;
         x[u] = from_a ? ia : ib;
         xa += from_a ? 1 : 0;
         yb += from_a ? 0 : 1;
+        const int p = from_a ? pad16(A0 + (xa < r ? xa : r - 1)) : pad16(B0 + (yb < r ? yb : r - 1));
+        const K nk = s_key[p];
+        const uint32_t ni = s_idx[p];
+        ka = from_a ? nk : ka;
+        ia = from_a ? ni : ia;
+        kb = from_a ? kb : nk;
+        ib = from_a ? ib : ni;
       }
     }
     __syncthreads();
@@ -575,23 +582,31 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     uint32_t ri[PER];
     const int o0 = tid * PER < n ? tid * PER : n;
     const int o1 = o0 + PER < n ? o0 + PER : n;
-    {
+    if (o0 < o1) {
+      // the heads of both runs stay in registers; each output reloads only
+      // the side it consumed (one key and one index read per output)
       int x = (int)merge_corank(s_key, s_idx, nal, s_key + nal, s_idx + nal, nbl, o0);
       int y = o0 - x;
+      const int last = n - 1;
+      int pa = x < last ? x : last, pb = nal + y < last ? nal + y : last;
+      K ka = s_key[pa], kb = s_key[pb];
+      uint32_t ia = s_idx[pa], ib = s_idx[pb];
 #pragma unroll
       for (int u = 0; u < PER; u++) {
         if (o0 + u < o1) {
-          const bool from_a =
-              y >= nbl || (x < nal && !pair_less(s_key[nal + y], s_idx[nal + y], s_key[x], s_idx[x]));
-          if (from_a) {
-            rk[u] = s_key[x];
-            ri[u] = s_idx[x];
-            x++;
-          } else {
-            rk[u] = s_key[nal + y];
-            ri[u] = s_idx[nal + y];
-            y++;
-          }
+          const bool from_a = y >= nbl || (x < nal && !pair_less(kb, ib, ka, ia));
+          rk[u] = from_a ? ka : kb;
+          ri[u] = from_a ? ia : ib;
+          x += from_a ? 1 : 0;
+          y += from_a ? 0 : 1;
+          int p = from_a ? x : nal + y;
+          p = p < last ? p : last;
+          const K nk = s_key[p];
+          const uint32_t ni = s_idx[p];
+          ka = from_a ? nk : ka;
+          ia = from_a ? ni : ia;
+          kb = from_a ? kb : nk;
+          ib = from_a ? ib : ni;
         }
       }
     }
