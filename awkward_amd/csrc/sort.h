@@ -157,6 +157,54 @@ __device__ void sort_wave(void* out, const T* from, int64_t a, int len, bool asc
   if (lane < len) store_sorted<T, ARG>(out, from, a, lane, idx);
 }
 
+// ---- (key, index) pairs in LDS -------------------------------------------------------
+// A 32-bit key and its index pack into one 64-bit word, key in the high half:
+// a pair moves in one LDS access and word order is pair order.  64-bit keys
+// keep two arrays.  A stage of `cap` pairs takes lds_pair_words<K>(cap)
+// 64-bit words.
+template <typename K>
+struct LdsPairs;
+
+template <>
+struct LdsPairs<uint32_t> {
+  uint64_t* w;
+  __device__ void get(int i, uint32_t& k, uint32_t& x) const {
+    const uint64_t v = w[i];
+    k = (uint32_t)(v >> 32);
+    x = (uint32_t)v;
+  }
+  __device__ void put(int i, uint32_t k, uint32_t x) const { w[i] = ((uint64_t)k << 32) | x; }
+  __device__ uint32_t idx(int i) const { return (uint32_t)w[i]; }
+  __device__ LdsPairs sub(int o) const { return LdsPairs{w + o}; }
+};
+
+template <>
+struct LdsPairs<uint64_t> {
+  uint64_t* k;
+  uint32_t* x;
+  __device__ void get(int i, uint64_t& kk, uint32_t& xx) const {
+    kk = k[i];
+    xx = x[i];
+  }
+  __device__ void put(int i, uint64_t kk, uint32_t xx) const {
+    k[i] = kk;
+    x[i] = xx;
+  }
+  __device__ uint32_t idx(int i) const { return x[i]; }
+  __device__ LdsPairs sub(int o) const { return LdsPairs{k + o, x + o}; }
+};
+
+template <typename K>
+constexpr int lds_pair_words(int cap) {
+  return sizeof(K) == 4 ? cap : cap + (cap + 1) / 2;
+}
+
+template <typename K>
+__device__ inline LdsPairs<K> lds_pairs(uint64_t* base, int cap) {
+  if constexpr (sizeof(K) == 4) return LdsPairs<K>{base};
+  else return LdsPairs<K>{base, (uint32_t*)(base + cap)};
+}
+
 // ---- len <= kCap: the workgroup, registers + LDS merges ------------------------------
 // LDS position of element i: one pad slot per 16 elements, so the 16-element
 // rows the lanes of a wave read and write fall in different banks.
@@ -170,22 +218,17 @@ constexpr int kCapPad = kCap + kCap / 16;
 // least 16).  Every thread of the workgroup calls it with the same np_sync
 // >= np (the merge stages and their barriers run up to np_sync), so several
 // groups of threads may sort different segments side by side.  The sorted
-// pairs end in s_key / s_idx at positions pad16(0 .. n-1).
+// pairs end in `sp` at positions pad16(0 .. n-1).
 template <typename T, int THREADS, typename K = KeyOf<T>>
-__device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_t a, int n, bool asc,
+__device__ void block_sort_part(const LdsPairs<K>& sp, const T* from, int64_t a, int n, bool asc,
                                 uint32_t idx_base, int t, int np_sync) {
   constexpr int PER = 16;
   int np = PER;
   while (np < n) np <<= 1;
   for (int i = t; i < np; i += THREADS) {
     const int p = pad16(i);
-    if (i < n) {
-      s_key[p] = (K)order_key<T>(from[a + i], asc);
-      s_idx[p] = idx_base + (uint32_t)i;
-    } else {
-      s_key[p] = ~K(0);
-      s_idx[p] = kPadIdx;
-    }
+    if (i < n) sp.put(p, (K)order_key<T>(from[a + i], asc), idx_base + (uint32_t)i);
+    else sp.put(p, ~K(0), kPadIdx);
   }
   __syncthreads();
   const bool active = t * PER < np;
@@ -194,16 +237,10 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
   const int o = t * PER;  // this thread's first position
   if (active) {
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-      k[u] = s_key[pad16(o + u)];
-      x[u] = s_idx[pad16(o + u)];
-    }
+    for (int u = 0; u < PER; u++) sp.get(pad16(o + u), k[u], x[u]);
     bitonic16(k, x);
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-      s_key[pad16(o + u)] = k[u];
-      s_idx[pad16(o + u)] = x[u];
-    }
+    for (int u = 0; u < PER; u++) sp.put(pad16(o + u), k[u], x[u]);
   }
   __syncthreads();
   for (int r = PER; r < np_sync; r <<= 1) {
@@ -217,15 +254,18 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
       while (lo < hi) {
         const int i = (lo + hi) >> 1;
         const int j = d - i;
-        const int pa = pad16(A0 + i), pb = pad16(B0 + j - 1);
-        if (pair_less(s_key[pa], s_idx[pa], s_key[pb], s_idx[pb])) lo = i + 1;
+        K ka, kb;
+        uint32_t ia, ib;
+        sp.get(pad16(A0 + i), ka, ia);
+        sp.get(pad16(B0 + j - 1), kb, ib);
+        if (pair_less(ka, ia, kb, ib)) lo = i + 1;
         else hi = i;
       }
       int xa = lo, yb = d - lo;
-      int pa = pad16(A0 + (xa < r ? xa : r - 1));
-      int pb = pad16(B0 + (yb < r ? yb : r - 1));
-      K ka = s_key[pa], kb = s_key[pb];
-      uint32_t ia = s_idx[pa], ib = s_idx[pb];
+      K ka, kb;
+      uint32_t ia, ib;
+      sp.get(pad16(A0 + (xa < r ? xa : r - 1)), ka, ia);
+      sp.get(pad16(B0 + (yb < r ? yb : r - 1)), kb, ib);
 #pragma unroll
       for (int u = 0; u < PER; u++) {
         const bool from_a = yb >= r || (xa < r && !pair_less(kb, ib, ka, ia));
@@ -234,8 +274,9 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
         xa += from_a ? 1 : 0;
         yb += from_a ? 0 : 1;
         const int p = from_a ? pad16(A0 + (xa < r ? xa : r - 1)) : pad16(B0 + (yb < r ? yb : r - 1));
-        const K nk = s_key[p];
-        const uint32_t ni = s_idx[p];
+        K nk;
+        uint32_t ni;
+        sp.get(p, nk, ni);
         ka = from_a ? nk : ka;
         ia = from_a ? ni : ia;
         kb = from_a ? kb : nk;
@@ -245,10 +286,7 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
     __syncthreads();
     if (merge) {
 #pragma unroll
-      for (int u = 0; u < PER; u++) {
-        s_key[pad16(o + u)] = k[u];
-        s_idx[pad16(o + u)] = x[u];
-      }
+      for (int u = 0; u < PER; u++) sp.put(pad16(o + u), k[u], x[u]);
     }
     __syncthreads();
   }
@@ -257,16 +295,20 @@ __device__ void block_sort_part(K* s_key, uint32_t* s_idx, const T* from, int64_
 // ---- longer: LDS runs + merge-path passes ------------------------------------------------
 // Number of elements taken from run A among the first `diag` outputs of the
 // stable merge of A (first) and B.
+// A holds pairs [0, na) of `sp`, B pairs [na, na + nb).
 template <typename K>
-__device__ inline int64_t merge_corank(const K* ka, const uint32_t* ia, int64_t na, const K* kb,
-                                       const uint32_t* ib, int64_t nb, int64_t diag) {
-  int64_t lo = diag > nb ? diag - nb : 0;
-  int64_t hi = diag < na ? diag : na;
+__device__ inline int merge_corank(const LdsPairs<K>& sp, int na, int nb, int diag) {
+  int lo = diag > nb ? diag - nb : 0;
+  int hi = diag < na ? diag : na;
   while (lo < hi) {
-    const int64_t i = (lo + hi) >> 1;  // take i from A, diag - i from B
-    const int64_t j = diag - i;
+    const int i = (lo + hi) >> 1;  // take i from A, diag - i from B
+    const int j = diag - i;
     // too few from A if A[i] < B[j-1]
-    if (j > 0 && i < na && pair_less(ka[i], ia[i], kb[j - 1], ib[j - 1])) lo = i + 1;
+    K ka, kb;
+    uint32_t ia, ib;
+    sp.get(i, ka, ia);
+    sp.get(na + j - 1, kb, ib);
+    if (pair_less(ka, ia, kb, ib)) lo = i + 1;
     else hi = i;
   }
   return lo;
@@ -397,9 +439,10 @@ __global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc
                                                            uint32_t* __restrict__ gi) {
   constexpr int NW = kBlock / kWave;
   constexpr int QUARTER = kCapPad / NW;  // pad16(kSmall) fits
-  __shared__ KeyOf<T> s_key[kCapPad];
-  __shared__ uint32_t s_idx[kCapPad];
+  using K = KeyOf<T>;
+  __shared__ uint64_t s_raw[lds_pair_words<K>(kCapPad)];
   __shared__ int s_np[NW];
+  const LdsPairs<K> sp = lds_pairs<K>(s_raw, kCapPad);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   int64_t ns = (int64_t)*q.small_count;
   if (ns > q.small_capacity) ns = q.small_capacity;
@@ -417,10 +460,9 @@ __global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc
     __syncthreads();
     int np_sync = s_np[0];
     for (int w = 1; w < NW; w++) np_sync = s_np[w] > np_sync ? s_np[w] : np_sync;
-    block_sort_part<T, kWave>(s_key + wave * QUARTER, s_idx + wave * QUARTER, (const T*)ch.from, ch.a, n, asc, 0,
-                              lane, np_sync);
+    block_sort_part<T, kWave>(sp.sub(wave * QUARTER), (const T*)ch.from, ch.a, n, asc, 0, lane, np_sync);
     for (int i = lane; i < n; i += kWave)
-      store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, s_idx[wave * QUARTER + pad16(i)]);
+      store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, sp.idx(wave * QUARTER + pad16(i)));
     __syncthreads();
   }
   int64_t nc = (int64_t)*q.count;
@@ -431,14 +473,17 @@ __global__ __launch_bounds__(kBlock) void long_runs_kernel(LongQueue q, bool asc
     const int n = (int)((ch.len - c0) < kCap ? (ch.len - c0) : kCap);
     int np = 16;
     while (np < n) np <<= 1;
-    block_sort_part<T, kBlock>(s_key, s_idx, (const T*)ch.from, ch.a + c0, n, asc, (uint32_t)c0, threadIdx.x, np);
+    block_sort_part<T, kBlock>(sp, (const T*)ch.from, ch.a + c0, n, asc, (uint32_t)c0, threadIdx.x, np);
     if (ch.len <= kCap) {
       for (int i = threadIdx.x; i < n; i += kBlock)
-        store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, s_idx[pad16(i)]);
+        store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, i, sp.idx(pad16(i)));
     } else {
       for (int i = threadIdx.x; i < n; i += kBlock) {
-        gk[ch.buf + c0 + i] = s_key[pad16(i)];
-        gi[ch.buf + c0 + i] = s_idx[pad16(i)];
+        K kk;
+        uint32_t xx;
+        sp.get(pad16(i), kk, xx);
+        gk[ch.buf + c0 + i] = kk;
+        gi[ch.buf + c0 + i] = xx;
       }
     }
     __syncthreads();
@@ -546,8 +591,8 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
   using K = KeyOf<T>;
   if ((int64_t)*maxlen <= w) return;  // every queued segment is already one run
   constexpr int PER = kCap / kBlock;   // outputs per thread
-  __shared__ K s_key[kCap];
-  __shared__ uint32_t s_idx[kCap];
+  __shared__ uint64_t s_raw[lds_pair_words<K>(kCap)];
+  const LdsPairs<K> sp = lds_pairs<K>(s_raw, kCap);
   int64_t nc = (int64_t)*count;
   if (nc > capacity) nc = capacity;
   const int tid = threadIdx.x;
@@ -568,13 +613,8 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     const int nal = (int)(x1 - x0), nbl = (int)(y1 - y0);
     const int n = nal + nbl;
     for (int i = tid; i < n; i += kBlock) {
-      if (i < nal) {
-        s_key[i] = ka[x0 + i];
-        s_idx[i] = ia[x0 + i];
-      } else {
-        s_key[i] = kb[y0 + i - nal];
-        s_idx[i] = ib[y0 + i - nal];
-      }
+      if (i < nal) sp.put(i, ka[x0 + i], ia[x0 + i]);
+      else sp.put(i, kb[y0 + i - nal], ib[y0 + i - nal]);
     }
     __syncthreads();
     // this thread's outputs [tid * PER, tid * PER + PER) of the block
@@ -585,12 +625,13 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     if (o0 < o1) {
       // the heads of both runs stay in registers; each output reloads only
       // the side it consumed (one key and one index read per output)
-      int x = (int)merge_corank(s_key, s_idx, nal, s_key + nal, s_idx + nal, nbl, o0);
+      int x = merge_corank(sp, nal, nbl, o0);
       int y = o0 - x;
       const int last = n - 1;
-      int pa = x < last ? x : last, pb = nal + y < last ? nal + y : last;
-      K ka = s_key[pa], kb = s_key[pb];
-      uint32_t ia = s_idx[pa], ib = s_idx[pb];
+      K ka, kb;
+      uint32_t ia, ib;
+      sp.get(x < last ? x : last, ka, ia);
+      sp.get(nal + y < last ? nal + y : last, kb, ib);
 #pragma unroll
       for (int u = 0; u < PER; u++) {
         if (o0 + u < o1) {
@@ -601,8 +642,9 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
           y += from_a ? 0 : 1;
           int p = from_a ? x : nal + y;
           p = p < last ? p : last;
-          const K nk = s_key[p];
-          const uint32_t ni = s_idx[p];
+          K nk;
+          uint32_t ni;
+          sp.get(p, nk, ni);
           ka = from_a ? nk : ka;
           ia = from_a ? ni : ia;
           kb = from_a ? kb : nk;
@@ -612,19 +654,18 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-      if (o0 + u < o1) {
-        s_key[o0 + u] = rk[u];
-        s_idx[o0 + u] = ri[u];
-      }
-    }
+    for (int u = 0; u < PER; u++)
+      if (o0 + u < o1) sp.put(o0 + u, rk[u], ri[u]);
     __syncthreads();
     if (2 * w >= ch.len) {
-      for (int i = tid; i < n; i += kBlock) store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, d0 + i, s_idx[i]);
+      for (int i = tid; i < n; i += kBlock) store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, d0 + i, sp.idx(i));
     } else {
       for (int i = tid; i < n; i += kBlock) {
-        dk[ch.buf + d0 + i] = s_key[i];
-        di[ch.buf + d0 + i] = s_idx[i];
+        K kk;
+        uint32_t xx;
+        sp.get(i, kk, xx);
+        dk[ch.buf + d0 + i] = kk;
+        di[ch.buf + d0 + i] = xx;
       }
     }
     __syncthreads();
