@@ -769,7 +769,12 @@ def reduce(layout, reducer, mask, keepdims, axis=-1):
             return out
         return IndexedOptionArray(outindex, out, layout.backend)
     if depth == 1:
-        raise NotImplementedError("the hip path reduces list arrays (a flat array's full reduction is not on it)")
+        # a flat array (negaxis == depth == 1): one list holding every element;
+        # the result has length 1 and the caller takes its value
+        nplike = layout.nplike
+        starts = nplike.asarray(np.zeros(1, dtype=np.int64))
+        offsets = nplike.asarray(np.array([0, layout.length], dtype=np.int64))
+        return layout._reduce_next(reducer, starts, offsets, 1, mask, False)
     if negaxis != 1 or depth > 2:
         nplike = layout.nplike
         starts = nplike.asarray(np.zeros(1, dtype=np.int64))

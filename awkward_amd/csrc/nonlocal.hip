@@ -18,6 +18,13 @@
 // t - segment start + (offset of group (b, 0)); its rank r within its
 // column group gives the rows of the bin before it that lack that column,
 // (row - first row of the bin) - r, which is nextshifts' `missing`.
+//
+// For maxcount <= 256 (the usual multiplicities) no sort is needed: the rank
+// of the element in column c of row r within its (bin, column) group is
+// S(r, c) - S(first row of the bin, c), where S(r, c) counts the rows before
+// r longer than c.  Rows go 256 to a workgroup, a row per thread; a wave
+// ballot per column gives the in-wave counts, a scan over the workgroups per
+// column the rest (colranks below).
 
 #include "errors.h"
 #include "scan.h"
@@ -389,6 +396,247 @@ __global__ void nummissing_kernel(int64_t* nummissing, const int64_t* hist, int6
   }
 }
 
+// ---- sort-free regrouping (maxcount <= kFastMax) ------------------------------------
+constexpr int kFastMax = 256;
+constexpr int kRowsPerGroup = kBlock;  // one row per thread
+
+// Shared by the passes over row groups: the group's rows, a row per thread,
+// with lengths clipped to [0, maxcount]; the per-wave counts of rows longer
+// than c (c below the group's longest row) in s_w, and that bound in s_ml.
+struct RowGroup {
+  int64_t r;    // this thread's row (valid when r < row1)
+  int64_t len;  // clipped length (0 past row1)
+  int ml;       // longest clipped length in the workgroup
+};
+
+__device__ inline RowGroup row_group(const int64_t* offsets, int64_t row0, int64_t row1, int64_t maxcount, int64_t j,
+                                     unsigned int (*s_w)[kFastMax], int* s_ml) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  RowGroup g;
+  g.r = row0 + j * kRowsPerGroup + threadIdx.x;
+  g.len = 0;
+  if (g.r < row1) {
+    const int64_t l = offsets[g.r + 1] - offsets[g.r];
+    g.len = l < 0 ? 0 : l > maxcount ? maxcount : l;
+  }
+  if (threadIdx.x == 0) *s_ml = 0;
+  __syncthreads();
+  if (g.len > 0) atomicMax(s_ml, (int)g.len);
+  __syncthreads();
+  g.ml = *s_ml;
+  for (int c = 0; c < g.ml; c++) {
+    const uint64_t bal = __ballot(g.len > c);
+    if (lane == 0) s_w[wave][c] = (unsigned int)__popcll(bal);
+  }
+  __syncthreads();
+  return g;
+}
+
+// chunkpre[c * (ngroups + 1) + j]: rows of group j longer than c (scanned in
+// place over j by colscan_kernel)
+__global__ __launch_bounds__(kBlock) void colgroup_kernel(int64_t* chunkpre, const int64_t* offsets, int64_t row0,
+                                                          int64_t row1, int64_t maxcount, int64_t ngroups) {
+  __shared__ unsigned int s_w[kBlock / kWave][kFastMax];
+  __shared__ int s_ml;
+  const int64_t j = blockIdx.x;
+  const RowGroup g = row_group(offsets, row0, row1, maxcount, j, s_w, &s_ml);
+  for (int c = threadIdx.x; c < maxcount; c += kBlock) {
+    int64_t t = 0;
+    if (c < g.ml)
+      for (int w = 0; w < kBlock / kWave; w++) t += s_w[w][c];
+    chunkpre[(int64_t)c * (ngroups + 1) + j] = t;
+  }
+}
+
+// per column c (one workgroup each): exclusive scan over the groups, the
+// total at position ngroups
+__global__ __launch_bounds__(kBlock) void colscan_kernel(int64_t* chunkpre, int64_t ngroups) {
+  __shared__ int64_t s_sum[kBlock];
+  int64_t* col = chunkpre + (int64_t)blockIdx.x * (ngroups + 1);
+  const int64_t per = (ngroups + kBlock - 1) / kBlock;
+  const int64_t a = (int64_t)threadIdx.x * per;
+  const int64_t b = a + per < ngroups ? a + per : ngroups;
+  int64_t local = 0;
+  for (int64_t i = a; i < b; i++) local += col[i];
+  s_sum[threadIdx.x] = local;
+  __syncthreads();
+  for (int d = 1; d < kBlock; d <<= 1) {
+    const int64_t v = threadIdx.x >= d ? s_sum[threadIdx.x - d] : 0;
+    __syncthreads();
+    s_sum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = s_sum[threadIdx.x] - local;
+  for (int64_t i = a; i < b; i++) {
+    const int64_t v = col[i];
+    col[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == kBlock - 1) col[ngroups] = s_sum[kBlock - 1];
+}
+
+// S(r, c) of this thread's row for column c < g.ml; calls ballot, so every
+// lane of the wave calls it with the same c
+__device__ inline int64_t rows_before(const RowGroup& g, int c, const int64_t* chunkpre, int64_t ngroups, int64_t j,
+                                      unsigned int (*s_w)[kFastMax]) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t bal = __ballot(g.len > c);
+  int64_t s = chunkpre[(int64_t)c * (ngroups + 1) + j] + __popcll(bal & ((1ull << lane) - 1ull));
+  for (int w = 0; w < wave; w++) s += s_w[w][c];
+  return s;
+}
+
+// binS[b * maxcount + c] = S(outer_offsets[b], c) for the bins starting at a
+// row of the group (several when bins are empty)
+__global__ __launch_bounds__(kBlock) void binstart_kernel(int64_t* binS, const int64_t* chunkpre,
+                                                          const int64_t* offsets, const int64_t* outer_offsets,
+                                                          int64_t outlength, int64_t row0, int64_t row1,
+                                                          int64_t maxcount, int64_t ngroups) {
+  __shared__ unsigned int s_w[kBlock / kWave][kFastMax];
+  __shared__ int s_ml;
+  const int64_t j = blockIdx.x;
+  const RowGroup g = row_group(offsets, row0, row1, maxcount, j, s_w, &s_ml);
+  int64_t b0 = outlength;  // first bin starting at this row (outlength: none)
+  if (g.r < row1) {
+    int64_t lo = 0, hi = outlength;  // first b with outer_offsets[b] >= r
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (outer_offsets[mid] < g.r) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < outlength && outer_offsets[lo] == g.r) b0 = lo;
+  }
+  // columns at or past the group's longest row: S(r, c) is the count before the group
+  for (int c = 0; c < (int)maxcount; c++) {
+    const int64_t s = c < g.ml ? rows_before(g, c, chunkpre, ngroups, j, s_w)
+                               : chunkpre[(int64_t)c * (ngroups + 1) + j];
+    for (int64_t b = b0; b < outlength && outer_offsets[b] == g.r; b++) binS[b * maxcount + c] = s;
+  }
+}
+
+// the bins starting at row1 (empty trailing bins, and b = outlength): every
+// row is before them
+__global__ __launch_bounds__(kBlock) void binstart_tail_kernel(int64_t* binS, const int64_t* chunkpre,
+                                                               const int64_t* outer_offsets, int64_t outlength,
+                                                               int64_t row1, int64_t maxcount, int64_t ngroups) {
+  const int64_t total = (outlength + 1) * maxcount;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t b = i / maxcount, c = i - b * maxcount;
+    if (outer_offsets[b] == row1) binS[i] = chunkpre[c * (ngroups + 1) + ngroups];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void cnt_from_bins_kernel(int64_t* cnt, const int64_t* binS, int64_t groups,
+                                                               int64_t maxcount) {
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += (int64_t)gridDim.x * kBlock)
+    cnt[g] = binS[g + maxcount] - binS[g];
+}
+
+// Every element (row r, column c < maxcount): its rank in group (bin, c) is
+// S(r, c) - binS[bin, c]; PLACE writes nextcarry, otherwise `missing`.
+template <bool PLACE>
+__global__ __launch_bounds__(kBlock) void colrank_scatter_kernel(int64_t* out, const int64_t* chunkpre,
+                                                                 const int64_t* binS, const int64_t* prefix,
+                                                                 const int64_t* offsets,
+                                                                 const int64_t* outer_offsets, int64_t outlength,
+                                                                 int64_t row0, int64_t row1, int64_t maxcount,
+                                                                 int64_t ngroups) {
+  __shared__ unsigned int s_w[kBlock / kWave][kFastMax];
+  __shared__ int s_ml;
+  const int64_t j = blockIdx.x;
+  const RowGroup g = row_group(offsets, row0, row1, maxcount, j, s_w, &s_ml);
+  if (g.ml == 0) return;  // workgroup-uniform
+  int64_t bin = 0, e0 = 0, bstart = 0;
+  if (g.len > 0) {
+    bin = upper_bound(outer_offsets, 0, outlength + 1, g.r) - 1;
+    e0 = offsets[g.r];
+    bstart = outer_offsets[bin];
+  }
+  for (int c = 0; c < g.ml; c++) {
+    const int64_t s = rows_before(g, c, chunkpre, ngroups, j, s_w);
+    if (c < g.len) {
+      const int64_t rank = s - binS[bin * maxcount + c];
+      if (PLACE) out[prefix[bin * maxcount + c] + rank] = e0 + c;
+      else out[e0 + c] = (g.r - bstart) - rank;
+    }
+  }
+}
+
+// The sort-free regrouping's tables, from the work-queue pool after the
+// ColMajor arrays: chunkpre (maxcount x (ngroups + 1)) and binS
+// ((outlength + 1) x maxcount).
+struct ColRanks {
+  int64_t ngroups;
+  int64_t* chunkpre;
+  int64_t* binS;
+};
+
+Error colranks(ColRanks& cr, const ColMajor& cm, const int64_t* offsets, const int64_t* outer_offsets,
+               int64_t outlength, int64_t maxcount, char* mem) {
+  hipStream_t st = stream();
+  cr.ngroups = ceil_div(cm.row1 - cm.row0 > 0 ? cm.row1 - cm.row0 : 0, (int64_t)kRowsPerGroup);
+  cr.chunkpre = (int64_t*)mem;
+  cr.binS = (int64_t*)(mem + round256((size_t)maxcount * (cr.ngroups + 1) * 8));
+  if (cr.ngroups > 0) {
+    colgroup_kernel<<<(unsigned)cr.ngroups, kBlock, 0, st>>>(cr.chunkpre, offsets, cm.row0, cm.row1, maxcount,
+                                                             cr.ngroups);
+  } else {
+    fill_kernel<<<grid_for(maxcount, kBlock, 8), kBlock, 0, st>>>(cr.chunkpre, maxcount, 0);  // ngroups + 1 == 1
+  }
+  colscan_kernel<<<(unsigned)maxcount, kBlock, 0, st>>>(cr.chunkpre, cr.ngroups);
+  if (cr.ngroups > 0)
+    binstart_kernel<<<(unsigned)cr.ngroups, kBlock, 0, st>>>(cr.binS, cr.chunkpre, offsets, outer_offsets, outlength,
+                                                             cm.row0, cm.row1, maxcount, cr.ngroups);
+  binstart_tail_kernel<<<grid_for((outlength + 1) * maxcount, kBlock, 8), kBlock, 0, st>>>(
+      cr.binS, cr.chunkpre, outer_offsets, outlength, cm.row1, maxcount, cr.ngroups);
+  AKHIP_LAUNCH_CHECK("colranks");
+  return success();
+}
+
+inline size_t colranks_bytes(int64_t rows, int64_t outlength, int64_t maxcount) {
+  const int64_t ngroups = ceil_div(rows > 0 ? rows : 0, (int64_t)kRowsPerGroup);
+  return round256((size_t)maxcount * (ngroups + 1) * 8) + round256((size_t)(outlength + 1) * maxcount * 8);
+}
+
+// Host values and the (bin, column) counts, prefix and maxgroup of `cm`
+// without the sort (keys and order stay unset).
+Error colmajor_fast(ColMajor& cm, ColRanks& cr, const int64_t* offsets, const int64_t* outer_offsets,
+                    int64_t outlength, int64_t maxcount, bool with_prefix) {
+  hipStream_t st = stream();
+  cm.row0 = read_device(outer_offsets, 0);
+  cm.row1 = read_device(outer_offsets, outlength);
+  cm.E0 = read_device(offsets, cm.row0);
+  cm.n = read_device(offsets, cm.row1) - cm.E0;
+  if (cm.n < 0) cm.n = 0;
+  cm.groups = outlength * maxcount;
+  const size_t gb = round256((size_t)cm.groups * 8), pb = round256((size_t)(cm.groups + 1) * 8);
+  char* q = (char*)queue_pool(256 + gb + pb + colranks_bytes(cm.row1 - cm.row0, outlength, maxcount));
+  if (!q) return failure("out of device memory for the column regrouping", kSliceNone, kSliceNone, kFile);
+  cm.maxgroup = (unsigned long long*)q;
+  cm.keys32 = nullptr;
+  cm.keys64 = nullptr;
+  cm.order = nullptr;
+  cm.segoff = nullptr;
+  cm.cnt = (int64_t*)(q + 256);
+  cm.prefix = (int64_t*)(q + 256 + gb);
+  Error e = colranks(cr, cm, offsets, outer_offsets, outlength, maxcount, q + 256 + gb + pb);
+  if (e.str) return e;
+  set_i64<<<1, 1, 0, st>>>((int64_t*)cm.maxgroup, 0);
+  set_i64<<<1, 1, 0, st>>>(cm.prefix, 0);
+  if (cm.groups > 0) {
+    cnt_from_bins_kernel<<<grid_for(cm.groups, kBlock, 8), kBlock, 0, st>>>(cm.cnt, cr.binS, cm.groups, maxcount);
+    if (with_prefix) {
+      const int64_t ntiles = ceil_div(cm.groups, (int64_t)256 * 8);
+      void* s = scratch(scan_scratch_bytes(ntiles));
+      if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
+      e = launch_lookback_scan<256, 8>(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
+      if (e.str) return e;
+    }
+  }
+  AKHIP_LAUNCH_CHECK("colmajor_fast");
+  return success();
+}
+
 }  // namespace
 }  // namespace akhip
 
@@ -444,9 +692,19 @@ Error awkward_ListOffsetArray_reduce_nonlocal_preparenext_64(int64_t* nextcarry,
     return success();
   }
   ColMajor cm;
-  Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
-  if (e.str) return e;
-  if (cm.n > 0) place_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, nextcarry, outlength, maxcount);
+  if (maxcount <= kFastMax) {
+    ColRanks cr;
+    Error e = colmajor_fast(cm, cr, offsets, outer_offsets, outlength, maxcount, true);
+    if (e.str) return e;
+    if (cr.ngroups > 0)
+      colrank_scatter_kernel<true><<<(unsigned)cr.ngroups, kBlock, 0, st>>>(
+          nextcarry, cr.chunkpre, cr.binS, cm.prefix, offsets, outer_offsets, outlength, cm.row0, cm.row1, maxcount,
+          cr.ngroups);
+  } else {
+    Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
+    if (e.str) return e;
+    if (cm.n > 0) place_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, nextcarry, outlength, maxcount);
+  }
   const int64_t hi = cm.groups + 1 > distinctslen ? cm.groups + 1 : distinctslen;
   groups_out_kernel<<<grid_for(hi, kBlock, 8), kBlock, 0, st>>>(cm, nextoffsets, distincts, distinctslen);
   maxgroup_out<<<1, 1, 0, st>>>(cm, maxnextparents);
@@ -463,11 +721,21 @@ Error awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64(int64_t* nummissing,
   hipStream_t st = stream();
   if (maxcount > 0) {
     ColMajor cm;
-    Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
-    if (e.str) return e;
-    if (cm.n > 0)
-      missing_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, missing, offsets, outer_offsets, outlength,
-                                                                   maxcount);
+    if (maxcount <= kFastMax) {
+      ColRanks cr;
+      Error e = colmajor_fast(cm, cr, offsets, outer_offsets, outlength, maxcount, false);
+      if (e.str) return e;
+      if (cr.ngroups > 0)
+        colrank_scatter_kernel<false><<<(unsigned)cr.ngroups, kBlock, 0, st>>>(
+            missing, cr.chunkpre, cr.binS, nullptr, offsets, outer_offsets, outlength, cm.row0, cm.row1, maxcount,
+            cr.ngroups);
+    } else {
+      Error e = colmajor(cm, offsets, outer_offsets, outlength, maxcount);
+      if (e.str) return e;
+      if (cm.n > 0)
+        missing_kernel<<<grid_for(cm.n, kBlock, 8), kBlock, 0, st>>>(cm, missing, offsets, outer_offsets, outlength,
+                                                                     maxcount);
+    }
     // nummissing: the last bin's short rows (histogram by length, then a running sum)
     int64_t* hist = cm.cnt;  // groups >= maxcount entries, free again after the scan
     fill_kernel<<<grid_for(maxcount, kBlock, 8), kBlock, 0, st>>>(hist, maxcount, 0);

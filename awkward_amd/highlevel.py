@@ -229,7 +229,10 @@ def _reduce(array, reducer, keepdims, mask_identity, axis):
     is the local path, higher axes reduce across lists by position."""
     if axis is None:
         raise NotImplementedError("axis=None (a full reduction) is not on the hip jagged-array path")
-    return Array(C.reduce(array.layout, reducer, mask_identity, keepdims, axis))
+    out = Array(C.reduce(array.layout, reducer, mask_identity, keepdims, axis))
+    if array.layout.depth == 1 and not keepdims:
+        return out.to_list()[0]  # a flat array reduces to one value (None: masked)
+    return out
 
 
 def sum(array, axis=-1, keepdims=False, mask_identity=False):

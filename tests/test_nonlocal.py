@@ -151,11 +151,14 @@ def test_three_levels_against_closed_form(make, axis):
 
 # ---- kernel level, GPU vs oracle ----------------------------------------------------
 @pytest.mark.gpu
-def test_nonlocal_kernels_large():
+@pytest.mark.parametrize("longest", [60, 600])
+def test_nonlocal_kernels_large(longest):
+    """maxcount <= 256 takes the sort-free regrouping (ballot counts per
+    column), longer rows the stable argsort by column."""
     rng = np.random.default_rng(41)
     length = 60000
     counts = rng.poisson(4, length)
-    counts[::501] = rng.integers(20, 60, len(counts[::501]))
+    counts[::501] = rng.integers(20, longest, len(counts[::501]))
     counts[rng.random(length) < 0.1] = 0
     offsets = offsets_from_counts(counts)
     cuts = np.sort(rng.integers(0, length + 1, 999))
