@@ -137,6 +137,7 @@ struct ColMajor {
   int64_t* cnt;        // groups: elements per (bin, column)
   int64_t* prefix;     // groups + 1: prefix[g] = elements of the groups before g
   unsigned long long* maxgroup;  // 1 + the last non-empty group (0: none)
+  int64_t last_bin_row;          // outer_offsets[outlength - 1] (host)
   __device__ int64_t key(int64_t t) const { return keys32 ? (int64_t)keys32[t] : keys64[t]; }
 };
 
@@ -251,6 +252,34 @@ __global__ __launch_bounds__(kBlock) void segoff_kernel(ColMajor cm, const int64
   if (b <= outlength) cm.segoff[b] = offsets[outer_offsets[b]] - cm.E0;
 }
 
+__global__ void bounds_kernel(int64_t* out, const int64_t* offsets, const int64_t* outer_offsets, int64_t outlength) {
+  const int64_t r0 = outer_offsets[0], r1 = outer_offsets[outlength];
+  out[0] = r0;
+  out[1] = r1;
+  out[2] = offsets[r0];
+  out[3] = offsets[r1];
+  out[4] = outer_offsets[outlength > 0 ? outlength - 1 : 0];
+}
+
+// The bins' row range, its element range and the last bin's first row, in
+// one host round trip.
+Error read_bounds(ColMajor& cm, const int64_t* offsets, const int64_t* outer_offsets, int64_t outlength) {
+  hipStream_t st = stream();
+  int64_t* d = (int64_t*)scratch(256);
+  if (!d) return failure("out of device memory for scratch", kSliceNone, kSliceNone, kFile);
+  int64_t h[5] = {0, 0, 0, 0, 0};
+  bounds_kernel<<<1, 1, 0, st>>>(d, offsets, outer_offsets, outlength);
+  hipError_t err = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err != hipSuccess) return hip_failure(err, "reading the column regrouping's bounds");
+  cm.row0 = h[0];
+  cm.row1 = h[1];
+  cm.E0 = h[2];
+  cm.n = h[3] - h[2];
+  cm.last_bin_row = h[4];
+  return success();
+}
+
 struct PrefixBody {
   const int64_t* cnt;
   int64_t* prefix;
@@ -268,10 +297,10 @@ struct PrefixBody {
 Error colmajor(ColMajor& cm, const int64_t* offsets, const int64_t* outer_offsets, int64_t outlength,
                int64_t maxcount) {
   hipStream_t st = stream();
-  cm.row0 = read_device(outer_offsets, 0);
-  cm.row1 = read_device(outer_offsets, outlength);
-  cm.E0 = read_device(offsets, cm.row0);
-  cm.n = read_device(offsets, cm.row1) - cm.E0;
+  {
+    Error e = read_bounds(cm, offsets, outer_offsets, outlength);
+    if (e.str) return e;
+  }
   if (cm.n < 0) cm.n = 0;
   cm.groups = outlength * maxcount;
   const bool narrow = maxcount < INT32_MAX;
@@ -603,10 +632,10 @@ inline size_t colranks_bytes(int64_t rows, int64_t outlength, int64_t maxcount) 
 Error colmajor_fast(ColMajor& cm, ColRanks& cr, const int64_t* offsets, const int64_t* outer_offsets,
                     int64_t outlength, int64_t maxcount, bool with_prefix) {
   hipStream_t st = stream();
-  cm.row0 = read_device(outer_offsets, 0);
-  cm.row1 = read_device(outer_offsets, outlength);
-  cm.E0 = read_device(offsets, cm.row0);
-  cm.n = read_device(offsets, cm.row1) - cm.E0;
+  {
+    Error e = read_bounds(cm, offsets, outer_offsets, outlength);
+    if (e.str) return e;
+  }
   if (cm.n < 0) cm.n = 0;
   cm.groups = outlength * maxcount;
   const size_t gb = round256((size_t)cm.groups * 8), pb = round256((size_t)(cm.groups + 1) * 8);
@@ -739,7 +768,7 @@ Error awkward_ListOffsetArray_reduce_nonlocal_nextshifts_64(int64_t* nummissing,
     // nummissing: the last bin's short rows (histogram by length, then a running sum)
     int64_t* hist = cm.cnt;  // groups >= maxcount entries, free again after the scan
     fill_kernel<<<grid_for(maxcount, kBlock, 8), kBlock, 0, st>>>(hist, maxcount, 0);
-    const int64_t r0 = read_device(outer_offsets, outlength - 1), r1 = cm.row1;
+    const int64_t r0 = cm.last_bin_row, r1 = cm.row1;
     if (r1 > r0) short_rows_kernel<<<grid_for(r1 - r0, kBlock, 8), kBlock, 0, st>>>(hist, offsets, r0, r1, maxcount);
     nummissing_kernel<<<1, 1, 0, st>>>(nummissing, hist, maxcount);
   }

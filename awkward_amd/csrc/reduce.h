@@ -294,8 +294,17 @@ __device__ inline bool queue_reduce(const ReduceQueue& q, bool want, int64_t bin
       if (first + c < q.capacity) q.chunks[first + c].bin = -1;
     return false;
   }
-  for (int64_t c = 0; c < (int64_t)nch; c++)
-    q.chunks[first + c] = ReduceChunk{bin, gb, ge, first, (int32_t)c, (int32_t)nch, item, 0};
+  // the wave writes each list's chunk entries together, a lane per entry
+  uint64_t lists = __ballot(want);
+  while (lists) {
+    const int src = __builtin_ctzll(lists);
+    lists &= lists - 1;
+    const int64_t f = __shfl(first, src, kWave);
+    const int64_t n = (int64_t)__shfl((long long)nch, src, kWave);
+    const int64_t lb = __shfl(bin, src, kWave), lgb = __shfl(gb, src, kWave), lge = __shfl(ge, src, kWave);
+    for (int64_t c = lane; c < n; c += kWave)
+      q.chunks[f + c] = ReduceChunk{lb, lgb, lge, f, (int32_t)c, (int32_t)n, item, 0};
+  }
   return want;
 }
 
@@ -589,13 +598,24 @@ __global__ __launch_bounds__(256) void segmented_reduce_queued_kernel(Op op, Epi
   reduce_tile<Op, 256, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength, (int64_t)blockIdx.x * 256, sm, q, 0);
 }
 
+template <int Dummy = 0>
+__global__ void span_kernel(int64_t* out, const int64_t* offsets, int64_t n) { out[0] = offsets[n] - offsets[0]; }
+
 template <class Op, class Epi>
 inline Error launch_reduce_queued(const Op& op, const Epi& epi, const typename Op::In* fromptr,
                                   const int64_t* offsets, int64_t outlength) {
   using Part = typename Op::Part;
   using In = typename Op::In;
   if (outlength <= 0) return success();
-  const int64_t elements = read_device(offsets, outlength) - read_device(offsets, 0);
+  int64_t elements = 0;
+  {
+    int64_t* d = (int64_t*)scratch(256);
+    if (!d) return failure("out of device memory for scratch", kSliceNone, kSliceNone, "awkward_amd/csrc/reduce.h");
+    span_kernel<><<<1, 1, 0, stream()>>>(d, offsets, outlength);
+    hipError_t err = hipMemcpyAsync(&elements, d, sizeof(int64_t), hipMemcpyDeviceToHost, stream());
+    if (err == hipSuccess) err = hipStreamSynchronize(stream());
+    if (err != hipSuccess) return hip_failure(err, "reading the element count");
+  }
   const int64_t cap = reduce_queue_capacity(elements > 0 ? elements : 1);
   const size_t cb = ((size_t)cap * sizeof(ReduceChunk) + 255) / 256 * 256;
   const size_t pb = ((size_t)cap * sizeof(Part) + 255) / 256 * 256;
