@@ -750,6 +750,25 @@ def _first(out):
     raise NotImplementedError(type(out).__name__)
 
 
+def completely_flatten(layout):
+    """The numbers of every list, in order, missing values dropped -- what
+    ``Content.completely_flatten`` gives the reference's axis=None reductions
+    (``_do.py:225-340``; ``contents/listoffsetarray.py``,
+    ``indexedoptionarray.py``, ``bytemaskedarray.py``, ``regulararray.py``)."""
+    if isinstance(layout, NumpyArray):
+        return layout
+    if isinstance(layout, (ListArray, RegularArray)):
+        layout = layout.to_ListOffsetArray64()
+    if isinstance(layout, ListOffsetArray):
+        return completely_flatten(layout._zero_based()[1])
+    if isinstance(layout, (IndexedOptionArray, ByteMaskedArray)):
+        nextcarry, _ = layout._project_valid()
+        return completely_flatten(layout.content._carry(nextcarry, False))
+    if isinstance(layout, IndexedArray):
+        return completely_flatten(layout.project())
+    raise TypeError(f"cannot reduce {type(layout).__name__} with axis=None (records have no single number type)")
+
+
 def reduce(layout, reducer, mask, keepdims, axis=-1):
     """_do.reduce (src/awkward/_do.py:225-340): returns the reduced layout
     (the outer length-1 level removed).  axis=-1 of a list of numbers (or of
@@ -801,5 +820,6 @@ __all__ = [
     "ListOffsetArray",
     "ListArray",
     "reduce",
+    "completely_flatten",
     "_reducers",
 ]

@@ -4,7 +4,7 @@ hipGraph capture of them.
 ``EventPipeline`` evaluates, for every event of a list-of-records array,
 
     sel = ev[(term_1) & (term_2) & ...][:, start:stop:step]
-    ak.num(sel), ak.sum(sel[field]), ak.argmax(sel[field]), ak.max(sel[field])
+    ak.num(sel), ak.sum(sel[field], axis=-1), ak.argmax(sel[field], axis=-1), ak.max(sel[field], axis=-1)
 
 in one kernel (``awkward_hip_event_pipeline``) with the same results as the
 per-kernel path; ``pair_max`` evaluates ak.max / ak.argmax over

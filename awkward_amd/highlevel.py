@@ -226,36 +226,45 @@ def _check_axis(axis):
 
 def _reduce(array, reducer, keepdims, mask_identity, axis):
     """Any axis of a list array (negative counts from the numbers); axis=-1
-    is the local path, higher axes reduce across lists by position."""
+    is the local path, higher axes reduce across lists by position, and
+    axis=None (the reference's default) reduces every number to one value
+    (positions of arg-reductions count in the flattened numbers, missing
+    values dropped)."""
+    layout = array.layout
     if axis is None:
-        raise NotImplementedError("axis=None (a full reduction) is not on the hip jagged-array path")
-    out = Array(C.reduce(array.layout, reducer, mask_identity, keepdims, axis))
-    if array.layout.depth == 1 and not keepdims:
+        if keepdims:
+            raise NotImplementedError("keepdims=True with axis=None is not on the hip jagged-array path")
+        layout = C.completely_flatten(layout)
+        axis = -1
+    out = Array(C.reduce(layout, reducer, mask_identity, keepdims, axis))
+    if layout.depth == 1 and not keepdims:
         return out.to_list()[0]  # a flat array reduces to one value (None: masked)
     return out
 
 
-def sum(array, axis=-1, keepdims=False, mask_identity=False):
+# Signatures of awkward.operations.ak_{sum,max,min,argmax,argmin,count}
+# (axis=None by default, as there).
+def sum(array, axis=None, keepdims=False, mask_identity=False):
     return _reduce(array, R.Sum(), keepdims, mask_identity, axis)
 
 
-def max(array, axis=-1, keepdims=False, initial=None, mask_identity=True):
+def max(array, axis=None, keepdims=False, initial=None, mask_identity=True):
     return _reduce(array, R.Max(initial), keepdims, mask_identity, axis)
 
 
-def min(array, axis=-1, keepdims=False, initial=None, mask_identity=True):
+def min(array, axis=None, keepdims=False, initial=None, mask_identity=True):
     return _reduce(array, R.Min(initial), keepdims, mask_identity, axis)
 
 
-def argmax(array, axis=-1, keepdims=False, mask_identity=True):
+def argmax(array, axis=None, keepdims=False, mask_identity=True):
     return _reduce(array, R.ArgMax(), keepdims, mask_identity, axis)
 
 
-def argmin(array, axis=-1, keepdims=False, mask_identity=True):
+def argmin(array, axis=None, keepdims=False, mask_identity=True):
     return _reduce(array, R.ArgMin(), keepdims, mask_identity, axis)
 
 
-def count(array, axis=-1, keepdims=False, mask_identity=False):
+def count(array, axis=None, keepdims=False, mask_identity=False):
     return _reduce(array, R.Count(), keepdims, mask_identity, axis)
 
 

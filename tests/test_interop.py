@@ -130,7 +130,7 @@ def test_to_buffers_device_container_is_zero_copy():
     back = ak.from_buffers(json.dumps(form), length, container, buffer_key="{form_key}/{attribute}")
     assert back.layout.offsets.data_ptr() == arr.layout.offsets.data_ptr()
     assert back.to_list() == arr.to_list()
-    assert ak.sum(back["pt"]).to_list() == ak.sum(arr["pt"]).to_list()
+    assert ak.sum(back["pt"], axis=-1).to_list() == ak.sum(arr["pt"], axis=-1).to_list()
 
 
 @pytest.mark.gpu
@@ -177,8 +177,9 @@ def test_from_buffers_host_bytes_and_narrow_offsets():
     hip = ak.from_buffers(form, 300, container)
     assert hip.layout.offsets.dtype.itemsize == 8
     ref = ak.from_offsets(off, data, OracleBackend())
-    for fn in (ak.max, ak.argmax, ak.count, ak.num):
-        assert fn(hip).to_list() == fn(ref).to_list(), fn.__name__
+    for fn in (ak.max, ak.argmax, ak.count):
+        assert fn(hip, axis=-1).to_list() == fn(ref, axis=-1).to_list(), fn.__name__
+    assert ak.num(hip).to_list() == ak.num(ref).to_list()
 
 
 @pytest.mark.gpu

@@ -40,12 +40,12 @@ PTP_ROWS = [[0, 1, None], [None, 3], [], None, [4, 5, None, 6]]
 def test_reference_literals_missing_values_and_lists(make):
     be = make()
     a = ak.from_list(PTP_ROWS, be)
-    assert ak.sum(a).to_list() == [1, 3, 0, None, 15]
-    assert ak.min(a).to_list() == [0, 3, None, None, 4]
-    assert ak.max(a).to_list() == [1, 3, None, None, 6]
-    assert ak.argmin(a).to_list() == [0, 1, None, None, 0]
-    assert ak.argmax(a).to_list() == [1, 1, None, None, 3]
-    assert ak.count(a).to_list() == [2, 1, 0, None, 3]
+    assert ak.sum(a, axis=-1).to_list() == [1, 3, 0, None, 15]
+    assert ak.min(a, axis=-1).to_list() == [0, 3, None, None, 4]
+    assert ak.max(a, axis=-1).to_list() == [1, 3, None, None, 6]
+    assert ak.argmin(a, axis=-1).to_list() == [0, 1, None, None, 0]
+    assert ak.argmax(a, axis=-1).to_list() == [1, 1, None, None, 3]
+    assert ak.count(a, axis=-1).to_list() == [2, 1, 0, None, 3]
 
 
 @pytest.mark.parametrize("make", BACKENDS)
@@ -54,9 +54,9 @@ def test_reference_literal_option_over_lists(make):
     nplike = be.nplike
     v2 = ak.from_list([[2.2, 1.1, 3.3], [], [4.4, 5.5], [5.5], [-4.4, -5.5, -6.6]], be).layout
     io = C.IndexedOptionArray(nplike.asarray(np.array([4, 3, -1, 4, 0], dtype=np.int64)), v2, be)
-    assert ak.argmin(ak.Array(io)).to_list() == [2, 0, None, 2, 1]
+    assert ak.argmin(ak.Array(io), axis=-1).to_list() == [2, 0, None, 2, 1]
     bm = C.ByteMaskedArray(nplike.asarray(np.array([0, 0, 1, 1, 0], dtype=np.int8)), v2, False, be)
-    assert ak.argmin(ak.Array(bm)).to_list() == [1, None, None, None, 2]
+    assert ak.argmin(ak.Array(bm), axis=-1).to_list() == [1, None, None, None, 2]
 
 
 def closed_forms(rows):
@@ -110,7 +110,7 @@ def test_reducers_against_closed_forms(make, kind):
     a = ak.from_list(rows, be, dtype=np.int64) if kind == "indexed" else as_bytemasked(rows, be)
     want = closed_forms(rows)
     for name in ("sum", "max", "min", "argmax", "argmin", "count"):
-        assert getattr(ak, name)(a).to_list() == want[name], name
+        assert getattr(ak, name)(a, axis=-1).to_list() == want[name], name
 
 
 # ---- kernel level, GPU vs oracle at sizes beyond one scan tile -----------------
@@ -207,6 +207,6 @@ def test_reducers_hip_vs_oracle_large(kind):
     results = {}
     for label, be in (("oracle", oracle_backend()), ("hip", hip_backend())):
         a = ak.from_list(rows, be, dtype=np.int64) if kind == "indexed" else as_bytemasked(rows, be)
-        results[label] = {name: getattr(ak, name)(a).to_list()
+        results[label] = {name: getattr(ak, name)(a, axis=-1).to_list()
                           for name in ("sum", "max", "min", "argmax", "argmin", "count")}
     assert results["hip"] == results["oracle"]

@@ -101,3 +101,21 @@ def test_nonlocal_axis0_empty_sublists(make):
     a = ak.from_list([[1.0, 2.0, 3.0], [], [6.0, 7.0]], make())
     assert ak.sum(a, axis=0).to_list() == [7.0, 9.0, 3.0]
     assert ak.count(a, axis=0).to_list() == [2, 2, 1]
+
+
+@pytest.mark.parametrize("make", BACKENDS)
+def test_axis_none_default(make):
+    """axis=None is the reference's default: every number reduced to one
+    value, missing values dropped, arg-reductions counting positions in the
+    flattened numbers (``_do.py:225-340`` with ``completely_flatten``)."""
+    be = make()
+    rows = [[3, None, 7], [], None, [7, 1]]
+    a = ak.from_list(rows, be, dtype=np.int64)
+    flat = [3, 7, 7, 1]
+    assert ak.sum(a) == 18 and ak.count(a) == 4
+    assert ak.max(a) == 7 and ak.argmax(a) == flat.index(7)
+    assert ak.min(a) == 1 and ak.argmin(a) == 3
+    nested = ak.from_list([[[1.5, 2.5], []], [[4.0]]], be)
+    assert ak.sum(nested) == 8.0 and ak.argmax(nested) == 2
+    empty = ak.from_list([[], []], be, dtype=np.float64)
+    assert ak.max(empty) is None and ak.sum(empty) == 0.0 and ak.count(empty) == 0
