@@ -224,6 +224,12 @@ def _check_axis(axis):
         raise NotImplementedError("the hip jagged-array path reduces at axis=-1 (lists of numbers)")
 
 
+def _out(result, highlevel):
+    """highlevel=False returns the layout, as the reference's operations do;
+    scalars are returned as they are."""
+    return result.layout if (not highlevel and isinstance(result, Array)) else result
+
+
 def _reduce(array, reducer, keepdims, mask_identity, axis):
     """Any axis of a list array (negative counts from the numbers); axis=-1
     is the local path, higher axes reduce across lists by position, and
@@ -244,31 +250,33 @@ def _reduce(array, reducer, keepdims, mask_identity, axis):
 
 # Signatures of awkward.operations.ak_{sum,max,min,argmax,argmin,count}
 # (axis=None by default, as there).
-def sum(array, axis=None, keepdims=False, mask_identity=False):
-    return _reduce(array, R.Sum(), keepdims, mask_identity, axis)
+def sum(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.Sum(), keepdims, mask_identity, axis), highlevel)
 
 
-def max(array, axis=None, keepdims=False, initial=None, mask_identity=True):
-    return _reduce(array, R.Max(initial), keepdims, mask_identity, axis)
+def max(array, axis=None, keepdims=False, initial=None, mask_identity=True, *, highlevel=True, behavior=None,
+        attrs=None):
+    return _out(_reduce(array, R.Max(initial), keepdims, mask_identity, axis), highlevel)
 
 
-def min(array, axis=None, keepdims=False, initial=None, mask_identity=True):
-    return _reduce(array, R.Min(initial), keepdims, mask_identity, axis)
+def min(array, axis=None, keepdims=False, initial=None, mask_identity=True, *, highlevel=True, behavior=None,
+        attrs=None):
+    return _out(_reduce(array, R.Min(initial), keepdims, mask_identity, axis), highlevel)
 
 
-def argmax(array, axis=None, keepdims=False, mask_identity=True):
-    return _reduce(array, R.ArgMax(), keepdims, mask_identity, axis)
+def argmax(array, axis=None, keepdims=False, mask_identity=True, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.ArgMax(), keepdims, mask_identity, axis), highlevel)
 
 
-def argmin(array, axis=None, keepdims=False, mask_identity=True):
-    return _reduce(array, R.ArgMin(), keepdims, mask_identity, axis)
+def argmin(array, axis=None, keepdims=False, mask_identity=True, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.ArgMin(), keepdims, mask_identity, axis), highlevel)
 
 
-def count(array, axis=None, keepdims=False, mask_identity=False):
-    return _reduce(array, R.Count(), keepdims, mask_identity, axis)
+def count(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.Count(), keepdims, mask_identity, axis), highlevel)
 
 
-def num(array, axis=1):
+def num(array, axis=1, *, highlevel=True, behavior=None, attrs=None):
     """List lengths (ak_num.py:133-142: stops - starts)."""
     if axis not in (1, -1):
         raise NotImplementedError("ak.num at axis=1 only")
@@ -282,18 +290,18 @@ def num(array, axis=1):
         raise NotImplementedError("ak.num of a non-list array")
     if nplike.dtype(starts) != np.dtype(np.int64):
         starts, stops = nplike.asarray(starts, np.int64), nplike.asarray(stops, np.int64)
-    return Array(C.NumpyArray(nplike.subtract(stops, starts), layout.backend))
+    return _out(Array(C.NumpyArray(nplike.subtract(stops, starts), layout.backend)), highlevel)
 
 
-def local_index(array, axis=-1):
+def local_index(array, axis=-1, *, highlevel=True, behavior=None, attrs=None):
     _check_axis(axis)
     layout = array.layout
     if isinstance(layout, C.ListArray):
         layout = layout.to_ListOffsetArray64()
-    return Array(layout._local_index())
+    return _out(Array(layout._local_index()), highlevel)
 
 
-def combinations(array, n, replacement=False, axis=1, fields=None):
+def combinations(array, n, replacement=False, axis=1, fields=None, *, highlevel=True, behavior=None, attrs=None):
     if axis not in (1, -1):
         raise NotImplementedError("ak.combinations at axis=1 only")
     layout = array.layout
@@ -301,23 +309,23 @@ def combinations(array, n, replacement=False, axis=1, fields=None):
         layout = layout.to_ListOffsetArray64()
     if fields is not None and len(fields) != n:
         raise ValueError("if provided, the length of 'fields' must be 'n'")
-    return Array(layout._combinations(n, replacement, fields))
+    return _out(Array(layout._combinations(n, replacement, fields)), highlevel)
 
 
-def argsort(array, axis=-1, ascending=True, stable=True):
+def argsort(array, axis=-1, ascending=True, stable=True, *, highlevel=True, behavior=None, attrs=None):
     _check_axis(axis)
     layout = array.layout
     if isinstance(layout, C.ListArray):
         layout = layout.to_ListOffsetArray64()
-    return Array(layout._argsort_local(ascending, stable))
+    return _out(Array(layout._argsort_local(ascending, stable)), highlevel)
 
 
-def sort(array, axis=-1, ascending=True, stable=True):
+def sort(array, axis=-1, ascending=True, stable=True, *, highlevel=True, behavior=None, attrs=None):
     _check_axis(axis)
     layout = array.layout
     if isinstance(layout, C.ListArray):
         layout = layout.to_ListOffsetArray64()
-    return Array(layout._sort_local(ascending, stable))
+    return _out(Array(layout._sort_local(ascending, stable)), highlevel)
 
 
 # ---- construction and backends ------------------------------------------------------
