@@ -432,8 +432,10 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
   const int64_t span = sm.off[nl] - E0;
 
   if (span <= CAP) {
-    // light tile: stage the span with coalesced loads, then serial loops
-    constexpr int U = 8;
+    // light tile: stage the span with coalesced loads, then serial loops.
+    // U loads per thread cover the whole stage (up to 16), so a span that
+    // fits is staged in one round trip to memory.
+    constexpr int U = (CAP + BLOCK - 1) / BLOCK < 16 ? (CAP + BLOCK - 1) / BLOCK : 16;
     for (int64_t b = 0; b < span; b += (int64_t)BLOCK * U) {
       In r[U];
 #pragma unroll
