@@ -33,28 +33,6 @@ Error take(void* out, const void* data, const int64_t* index, int64_t n) {
   return success();
 }
 
-struct BoolToIndexBody {
-  int64_t* toindex;
-  int64_t* tooffsets;
-  const bool* mask;
-  const int64_t* offsets;
-  int64_t base;  // offsets[0] is read on the device: positions are relative to it
-  __device__ int64_t count(int64_t i) const {
-    int64_t c = 0;
-    for (int64_t j = offsets[i]; j < offsets[i + 1]; j++) c += mask[j] ? 1 : 0;
-    return c;
-  }
-  __device__ void emit(int64_t i, int64_t excl, int64_t cnt) const {
-    if (i == 0) tooffsets[0] = 0;
-    tooffsets[i + 1] = excl + cnt;
-    if (cnt == 0) return;
-    const int64_t a = offsets[i], b = offsets[i + 1];
-    int64_t k = excl;
-    for (int64_t j = a; j < b; j++)
-      if (mask[j]) toindex[k++] = j - a;
-  }
-};
-
 __global__ void sub_kernel(int64_t* __restrict__ out, const int64_t* __restrict__ a, const int64_t* __restrict__ b,
                            int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -75,6 +53,110 @@ __global__ __launch_bounds__(256) void stream_read_kernel(const uint4* __restric
   const uint4 v = p[i];
   const uint64_t x = ((uint64_t)(v.x ^ v.z) << 32) | (uint64_t)(v.y ^ v.w);
   if (x == key) sink[0] = x;
+}
+
+// A look-back scan over lists (scan.h's structure) whose tile first stages
+// its mask bytes in LDS (4-byte coalesced loads of the tile's element span),
+// so the per-list count and emit loops read LDS instead of walking global
+// memory byte by byte; a tile whose span exceeds the stage reads global
+// memory.  count(list) = selected elements; emit writes tooffsets[i + 1] and
+// the selected elements' local positions.
+constexpr int kB2IBlock = 256;
+constexpr int kB2IIPT = 4;
+constexpr int kB2ITile = kB2IBlock * kB2IIPT;  // lists per tile
+constexpr int kB2IWords = 2048;                // 8 KiB of mask bytes
+__global__ __launch_bounds__(kB2IBlock) void bool_to_index_kernel(int64_t* __restrict__ toindex,
+                                                                  int64_t* __restrict__ tooffsets,
+                                                                  const bool* __restrict__ mask,
+                                                                  const int64_t* __restrict__ offsets, int64_t n,
+                                                                  ScanScratch s, uint64_t* errword, uint32_t inv) {
+  constexpr int NW = kB2IBlock / kWave;
+  __shared__ int64_t s_off[kB2ITile + 1];
+  __shared__ uint32_t s_mask[kB2IWords];
+  __shared__ int64_t s_v[kB2ITile + kB2ITile / 32];
+  __shared__ int64_t s_wave[NW];
+  __shared__ int64_t s_prefix;
+  __shared__ unsigned s_tile;
+  auto pad = [](int p) { return p + (p >> 5); };
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+
+  if (tid == 0) s_tile = atomicAdd(s.counter, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t L0 = tile * (int64_t)kB2ITile;
+  const int nl = (int)((n - L0) < kB2ITile ? (n - L0) : kB2ITile);
+  for (int t = tid; t <= nl; t += kB2IBlock) s_off[t] = offsets[L0 + t];
+  __syncthreads();
+  // the span's bytes, from the 4-byte word holding its first byte
+  const uintptr_t first = (uintptr_t)(mask + s_off[0]);
+  const uintptr_t abase = first & ~(uintptr_t)3;
+  const int64_t words = (int64_t)(((uintptr_t)(mask + s_off[nl]) + 3 - abase) >> 2);
+  const bool staged = s_off[nl] >= s_off[0] && words <= kB2IWords;
+  if (staged)
+    for (int w = tid; w < (int)words; w += kB2IBlock) s_mask[w] = ((const uint32_t*)abase)[w];
+  __syncthreads();
+  const uint8_t* lmask = (const uint8_t*)s_mask;
+  auto selected = [&](int64_t j) -> bool {
+    return staged ? lmask[(uintptr_t)(mask + j) - abase] != 0 : mask[j];
+  };
+
+  int64_t c[kB2IIPT];
+#pragma unroll
+  for (int u = 0; u < kB2IIPT; u++) {
+    const int t = u * kB2IBlock + tid;
+    int64_t cnt = 0;
+    if (t < nl)
+      for (int64_t j = s_off[t]; j < s_off[t + 1]; j++) cnt += selected(j) ? 1 : 0;
+    c[u] = cnt;
+    s_v[pad(t)] = cnt;
+  }
+  __syncthreads();
+  int64_t b[kB2IIPT];
+  int64_t total = 0;
+#pragma unroll
+  for (int v = 0; v < kB2IIPT; v++) {
+    b[v] = s_v[pad(tid * kB2IIPT + v)];
+    total += b[v];
+  }
+  const int64_t incl = wave_inclusive_scan(total);
+  if (lane == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  int64_t wave_off = 0, block_total = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    const int64_t v = s_wave[w];
+    wave_off += (w < wave) ? v : 0;
+    block_total += v;
+  }
+  if (wave == 0) {
+    const int64_t prefix = lookback_prefix(s, tile, block_total, errword, inv);
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  int64_t run = s_prefix + wave_off + incl - total;
+#pragma unroll
+  for (int v = 0; v < kB2IIPT; v++) {
+    s_v[pad(tid * kB2IIPT + v)] = run;
+    run += b[v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kB2IIPT; u++) {
+    const int t = u * kB2IBlock + tid;
+    if (t < nl) {
+      const int64_t i = L0 + t;
+      const int64_t excl = s_v[pad(t)];
+      if (i == 0) tooffsets[0] = 0;
+      tooffsets[i + 1] = excl + c[u];
+      int64_t k = excl;
+      const int64_t a = s_off[t];
+      if (c[u] > 0)
+        for (int64_t j = a; j < s_off[t + 1]; j++)
+          if (selected(j)) toindex[k++] = j - a;
+    }
+  }
 }
 
 }  // namespace
@@ -106,11 +188,16 @@ Error awkward_hip_bool_to_index(int64_t* toindex, int64_t* tooffsets, const bool
     AKHIP_LAUNCH_CHECK("zero1");
     return success();
   }
-  const int64_t ntiles = ceil_div(n, (int64_t)256 * 8);
-  void* s = scratch(scan_scratch_bytes(ntiles));
-  if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
-  BoolToIndexBody body{toindex, tooffsets, mask, offsets, 0};
-  return launch_lookback_scan<256, 8>(body, n, 0, s);
+  const int64_t ntiles = ceil_div(n, (int64_t)kB2ITile);
+  void* base = scratch(scan_scratch_bytes(ntiles));
+  if (!base) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
+  const ScanScratch s = scan_scratch_at(base);
+  hipStream_t st = stream();
+  scan_prep_kernel<0><<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(s, ntiles);
+  bool_to_index_kernel<<<(unsigned)ntiles, kB2IBlock, 0, st>>>(toindex, tooffsets, mask, offsets, n, s, error_word(),
+                                                               0);
+  AKHIP_LAUNCH_CHECK("bool_to_index_kernel");
+  return success();
 }
 
 // Measurement: reads the first nbytes / 16 * 16 bytes of `data` (16-byte

@@ -127,3 +127,50 @@ def test_argsort_sort_api(backends, ascending):
         h = fn(ak.from_offsets(off, x, hip), ascending=ascending)
         for a, b in zip(flat(c), flat(h)):
             assert same_bits(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["poisson", "long_lists", "offset_start", "unaligned_mask", "all_false", "all_true",
+                                  "empty_lists"])
+def test_bool_to_index_kernel(case):
+    """awkward_hip_bool_to_index (tiles of 1024 lists whose mask bytes are
+    staged in LDS, or read from global memory when a tile's span exceeds the
+    8 KiB stage) against numpy: the local positions of the true values per
+    list and their offsets (the reference's _slicing.py:476-530 steps)."""
+    import torch
+
+    from awkward_amd.nplike import TorchHip
+
+    rng = np.random.default_rng(77)
+    n = 50_000
+    counts = rng.poisson(3, n)
+    if case == "long_lists":
+        sel = rng.random(n) < 0.3
+        counts[sel] = rng.integers(15, 60, int(sel.sum()))
+        counts[::997] = 5000
+    if case == "empty_lists":
+        counts[rng.random(n) < 0.7] = 0
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    total = int(offsets[-1])
+    mask = rng.random(total + 7) < 0.4
+    if case == "all_false":
+        mask[:] = False
+    if case == "all_true":
+        mask[:] = True
+    if case == "offset_start":
+        offsets = offsets + 5  # the lists start 5 bytes into the mask
+    big = torch.from_numpy(mask).cuda()
+    d_mask = big[3:] if case == "unaligned_mask" else big
+    host_mask = mask[3:] if case == "unaligned_mask" else mask
+    d_off = torch.from_numpy(offsets).cuda()
+    nl = TorchHip()
+    index, tooffsets = nl.bool_to_jagged_index(d_off, d_mask)
+    want_index, want_off = [], [0]
+    for a, b in zip(offsets[:-1], offsets[1:]):
+        sel = np.nonzero(host_mask[a:b])[0]
+        want_index.append(sel)
+        want_off.append(want_off[-1] + len(sel))
+    want_index = np.concatenate(want_index) if want_index else np.zeros(0, np.int64)
+    assert np.array_equal(tooffsets.cpu().numpy(), np.array(want_off, dtype=np.int64)), case
+    assert np.array_equal(index.cpu().numpy(), want_index.astype(np.int64)), case
