@@ -14,7 +14,7 @@
 // awkward_reduce_argmax + adjust_starts + reduce_mask.  Here a workgroup of
 // events loads the float32 fields of its particles in one coalesced pass,
 // evaluates the predicate on the registers they land in and keeps only the
-// reduced field and a flag per particle in LDS; then each thread walks its
+// reduced value and its flag per particle in LDS (one 8-byte word); then each thread walks its
 // event: once to count the selected particles (the slice is regularised
 // against that count, as awkward_regularize_rangeslice does) and once to
 // reduce the ones the slice keeps, in particle order -- the same serial
@@ -125,7 +125,7 @@ struct PipelineOut {
 
 // Staging plan: slot[k] >= 0 for the float32 fields loaded by the staging
 // pass (-1: read from global memory where a term needs it); the LDS stage
-// holds cap floats of the reduced field and cap flag bytes.
+// holds cap (reduced value, predicate flag) pairs of 8 bytes.
 struct Stage {
   int slot[kMaxFields];
   int nslots;
@@ -172,7 +172,8 @@ __device__ inline bool keep_particle(const TermRT* rt, int64_t i, int64_t local)
 // BLOCK threads, LPT events each (thread t owns events t + j * BLOCK, so
 // output stores coalesce); the workgroup's particles are loaded with
 // coalesced loads -- every float32 field's loads issued together -- and
-// staged (reduced field + predicate flags) in an LDS stage sized at launch
+// staged (reduced value and predicate flag, one 8-byte word per particle,
+// so each particle costs one LDS read in the per-event loops) in an LDS stage sized at launch
 // (dynamic LDS), to keep many loads in flight per CU.
 template <int BLOCK, int LPT, int NT>
 __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
@@ -218,10 +219,9 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
   }
   // staged: the float32 fields' particles are loaded with coalesced loads;
   // the predicate is evaluated on the registers they land in (terms on other
-  // fields read global memory), and only the reduced field and one flag byte
-  // per particle go to LDS
-  float* red_row = s_stage;
-  uint8_t* s_flag = (uint8_t*)(s_stage + st.cap);
+  // fields read global memory), and only the reduced value and its flag
+  // go to LDS, one word per particle
+  uint2* s_pv = (uint2*)s_stage;  // (reduced value bits, predicate flag) per particle
   if (staged) {
     const float* src[kMaxFields];
 #pragma unroll
@@ -258,18 +258,18 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
               keep = eval_rt(rt[t], E0 + q, q);
             }
           }
-          s_flag[q] = keep ? 1 : 0;
+          const bool kept = keep;
           float v = 0.0f;
 #pragma unroll
           for (int k = 0; k < kMaxFields; k++)
             if (k == red) v = r[k][u];
-          red_row[q] = v;
+          s_pv[q] = make_uint2(__float_as_uint(v), kept ? 1u : 0u);
         }
       }
     }
   }
   __syncthreads();
-  const float* red_lds = staged ? red_row : nullptr;
+
   const float* red_g = (const float*)fields.ptr[red];
   // [start:] with step 1 (start >= 0, or omitted): the kept particles are the
   // selected ones of rank >= start, known without the count -- one pass
@@ -285,9 +285,10 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
       float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
       int64_t best = -1, t = 0, sel = 0;
       for (int q = (int)(a - E0); q < (int)(b - E0); q++) {
-        if (!s_flag[q]) continue;
+        const uint2 w = s_pv[q];
+        if (!w.y) continue;
         if (sel >= start0) {
-          const float v = red_lds ? red_lds[q] : red_g[E0 + q];
+          const float v = __uint_as_float(w.x);
           sum += v;
           if (t == 0) {
             best = 0;
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
     // pass 1: number of selected particles
     int64_t m = 0;
     if (staged) {
-      for (int q = (int)(a - E0); q < (int)(b - E0); q++) m += s_flag[q];
+      for (int q = (int)(a - E0); q < (int)(b - E0); q++) m += s_pv[q].y;
     } else {
       for (int64_t i = a; i < b; i++) m += keep_particle<NT>(rt, i, i - E0) ? 1 : 0;
     }
@@ -323,10 +324,10 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
     int64_t best = -1, t = 0, sel = 0;
     int64_t next = first;
     for (int64_t i = a; i < b && t < cnt; i++) {
-      const bool keep = staged ? s_flag[i - E0] != 0 : keep_particle<NT>(rt, i, i - E0);
+      const bool keep = staged ? s_pv[i - E0].y != 0 : keep_particle<NT>(rt, i, i - E0);
       if (!keep) continue;
       if (sel == next) {
-        const float v = red_lds ? red_lds[i - E0] : red_g[i];
+        const float v = staged ? __uint_as_float(s_pv[i - E0].x) : red_g[i];
         sum += v;
         if (t == 0) {
           best = 0;
@@ -561,9 +562,9 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_pa
   while (lpt < 8 && (double)B * (2 * lpt) * per_event <= 4096.0) lpt *= 2;
   int64_t cap = (int64_t)(B * lpt * mean * 1.5) + 256;
   cap = (cap + 63) / 64 * 64;
-  if (cap > 12288) cap = 12288;
+  if (cap > 8192) cap = 8192;  // 8-byte (value, flag) pairs: at most 64 KiB of LDS
   st.cap = (int)cap;
-  const size_t lds = (size_t)st.cap * sizeof(float) + (size_t)st.cap;  // the reduced field + predicate flags
+  const size_t lds = (size_t)st.cap * 8;  // (reduced value, predicate flag) pairs
   const unsigned grid = (unsigned)ceil_div(n, (int64_t)B * lpt);
   hipStream_t s = stream();
   switch (lpt * 16 + nterms) {
