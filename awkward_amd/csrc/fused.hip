@@ -183,11 +183,22 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
   constexpr int L = BLOCK * LPT;
   constexpr int U = 4;
   extern __shared__ float s_stage[];
-  __shared__ int64_t s_off[L + 1];
+  // one wave, one event per lane: each lane holds its event's bounds in
+  // registers (no LDS round trip for the offsets)
+  constexpr bool kRegs = BLOCK == kWave && LPT == 1;
+  __shared__ int64_t s_off[kRegs ? 1 : L + 1];
   const int tid = threadIdx.x;
   const int64_t L0 = (int64_t)blockIdx.x * L;
   const int nl = (int)((n - L0) < L ? (n - L0) : L);
-  {
+  int64_t own_a = 0, own_b = 0, E0, span;
+  if constexpr (kRegs) {
+    if (tid < nl) {
+      own_a = offsets[L0 + tid];
+      own_b = offsets[L0 + tid + 1];
+    }
+    E0 = __shfl(own_a, 0, kWave);
+    span = __shfl(own_b, nl - 1, kWave) - E0;
+  } else {
     int64_t r[LPT + 1];
 #pragma unroll
     for (int j = 0; j <= LPT; j++) {
@@ -199,10 +210,10 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
       const int t = tid + j * BLOCK;
       if (t <= nl) s_off[t] = r[j];
     }
+    __syncthreads();
+    E0 = s_off[0];
+    span = s_off[nl] - E0;
   }
-  __syncthreads();
-  const int64_t E0 = s_off[0];
-  const int64_t span = s_off[nl] - E0;
   const bool staged = span <= st.cap;
   TermRT rt[NT > 0 ? NT : 1];
 #pragma unroll
@@ -280,7 +291,14 @@ __global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __
     const int e = tid + j * BLOCK;
     if (e >= nl) break;
     const int64_t ev = L0 + e;
-    const int64_t a = s_off[e], b = s_off[e + 1];
+    int64_t a, b;
+    if constexpr (kRegs) {
+      a = own_a;
+      b = own_b;
+    } else {
+      a = s_off[e];
+      b = s_off[e + 1];
+    }
     if (one_pass) {
       float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
       int64_t best = -1, t = 0, sel = 0;
