@@ -414,7 +414,19 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
   const int wave = tid / kWave;
   const int nl = (int)((outlength - L0) < L ? (outlength - L0) : L);
 
-  {
+  // one wave, one list per lane: each lane holds its list's bounds in
+  // registers and the tile's come by shuffles, so a light tile never passes
+  // its offsets through LDS (the heavy path below still stages them)
+  constexpr bool kRegs = BLOCK == kWave && LPT == 1;
+  int64_t own_a = 0, own_b = 0, E0, span;
+  if constexpr (kRegs) {
+    if (tid < nl) {
+      own_a = offsets[L0 + tid];
+      own_b = offsets[L0 + tid + 1];
+    }
+    E0 = __shfl(own_a, 0, kWave);
+    span = __shfl(own_b, nl - 1, kWave) - E0;
+  } else {
     int64_t r[LPT + 1];
 #pragma unroll
     for (int j = 0; j <= LPT; j++) {
@@ -426,10 +438,10 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
       const int t = tid + j * BLOCK;
       if (t <= nl) sm.off[t] = r[j];
     }
+    __syncthreads();
+    E0 = sm.off[0];
+    span = sm.off[nl] - E0;
   }
-  __syncthreads();
-  const int64_t E0 = sm.off[0];
-  const int64_t span = sm.off[nl] - E0;
 
   if (span <= CAP) {
     // light tile: stage the span with coalesced loads, then serial loops.
@@ -454,7 +466,7 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
     for (int j = 0; j < LPT; j++) {
       const int t = tid + j * BLOCK;
       if (t < nl) {
-        const int64_t gb = sm.off[t], ge = sm.off[t + 1];
+        const int64_t gb = kRegs ? own_a : sm.off[t], ge = kRegs ? own_b : sm.off[t + 1];
         epi.store(L0 + t, op.sequential(sm.data, gb - E0, ge - E0, gb), gb, ge);
       }
     }
@@ -463,6 +475,11 @@ __device__ inline void reduce_tile(const Op& op, const Epi& epi, const typename 
   }
 
   // heavy tile
+  if constexpr (kRegs) {
+    if (tid < nl) sm.off[tid] = own_a;
+    if (tid == nl - 1) sm.off[nl] = own_b;
+    __syncthreads();
+  }
 #pragma unroll
   for (int j = 0; j < LPT; j++) {
     const int t = tid + j * BLOCK;
