@@ -6,6 +6,7 @@ The kernels are hand-written HIP for gfx950 in ``libawkward_hip.so``
 ``HipBackend`` (``backend[(name, *dtypes)](*args)``), and driven by a host
 layer that issues the reference's call sequences for ``sum``, ``max``,
 ``min``, ``argmax``, ``argmin``, ``count``, ``num``, ``combinations``,
+``argcombinations``,
 jagged and range ``__getitem__``, ``sort`` and ``argsort`` at axis=-1.
 Buffers enter and leave without copies through DLPack (``from_dlpack``)
 and the Form/length/container triple (``to_buffers`` / ``from_buffers``).
@@ -13,6 +14,7 @@ and the Form/length/container triple (``to_buffers`` / ``from_buffers``).
 
 from awkward_amd.highlevel import (  # noqa: F401
     Array,
+    argcombinations,
     argmax,
     argmin,
     argsort,

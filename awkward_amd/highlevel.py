@@ -296,7 +296,7 @@ def num(array, axis=1, *, highlevel=True, behavior=None, attrs=None):
 def local_index(array, axis=-1, *, highlevel=True, behavior=None, attrs=None):
     _check_axis(axis)
     layout = array.layout
-    if isinstance(layout, C.ListArray):
+    if isinstance(layout, (C.ListArray, C.RegularArray)):
         layout = layout.to_ListOffsetArray64()
     return _out(Array(layout._local_index()), highlevel)
 
@@ -305,11 +305,21 @@ def combinations(array, n, replacement=False, axis=1, fields=None, *, highlevel=
     if axis not in (1, -1):
         raise NotImplementedError("ak.combinations at axis=1 only")
     layout = array.layout
-    if isinstance(layout, C.ListArray):
+    if isinstance(layout, (C.ListArray, C.RegularArray)):
         layout = layout.to_ListOffsetArray64()
     if fields is not None and len(fields) != n:
         raise ValueError("if provided, the length of 'fields' must be 'n'")
     return _out(Array(layout._combinations(n, replacement, fields)), highlevel)
+
+
+def argcombinations(array, n, replacement=False, axis=1, fields=None, *, highlevel=True, behavior=None,
+                    attrs=None):
+    """ak.argcombinations (operations/ak_argcombinations.py): the combinations
+    of each list's local indices."""
+    if axis not in (1, -1):
+        raise NotImplementedError("ak.argcombinations at axis=1 only")
+    return combinations(local_index(array, axis=-1), n, replacement=replacement, axis=axis, fields=fields,
+                        highlevel=highlevel)
 
 
 def argsort(array, axis=-1, ascending=True, stable=True, *, highlevel=True, behavior=None, attrs=None):
