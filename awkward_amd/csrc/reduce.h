@@ -91,8 +91,17 @@ struct SumOp {
   using Out = OUT;
   using Part = OUT;
   __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    // the CPU loop's order; four loads issued ahead of their adds
     OUT acc = OUT(0);
-    for (int64_t i = b; i < e; i++) acc = wrap_add<OUT>(acc, (OUT)p[i]);
+    int64_t i = b;
+    for (; i + 4 <= e; i += 4) {
+      const In v0 = p[i], v1 = p[i + 1], v2 = p[i + 2], v3 = p[i + 3];
+      acc = wrap_add<OUT>(acc, (OUT)v0);
+      acc = wrap_add<OUT>(acc, (OUT)v1);
+      acc = wrap_add<OUT>(acc, (OUT)v2);
+      acc = wrap_add<OUT>(acc, (OUT)v3);
+    }
+    for (; i < e; i++) acc = wrap_add<OUT>(acc, (OUT)p[i]);
     return acc;
   }
   __device__ Part init() const { return OUT(0); }
@@ -154,7 +163,13 @@ struct ExtremumOp {
   T identity;
   __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
     T best = identity;
-    for (int64_t i = b; i < e; i++) {
+    int64_t i = b;
+    for (; i + 4 <= e; i += 4) {
+      const T v[4] = {p[i], p[i + 1], p[i + 2], p[i + 3]};
+#pragma unroll
+      for (int u = 0; u < 4; u++) best = (MAX ? (v[u] > best) : (v[u] < best)) ? v[u] : best;
+    }
+    for (; i < e; i++) {
       const T v = p[i];
       best = (MAX ? (v > best) : (v < best)) ? v : best;
     }
@@ -181,7 +196,17 @@ struct ArgOp {
     if (b >= e) return -1;
     int64_t best = b;
     T best_val = p[b];
-    for (int64_t i = b + 1; i < e; i++) {
+    int64_t i = b + 1;
+    for (; i + 4 <= e; i += 4) {
+      const T v[4] = {p[i], p[i + 1], p[i + 2], p[i + 3]};
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (MAX ? (v[u] > best_val) : (v[u] < best_val)) {
+          best_val = v[u];
+          best = i + u;
+        }
+    }
+    for (; i < e; i++) {
       const T v = p[i];
       if (MAX ? (v > best_val) : (v < best_val)) {
         best_val = v;
