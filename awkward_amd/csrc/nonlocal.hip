@@ -535,6 +535,7 @@ __global__ __launch_bounds__(kBlock) void binstart_kernel(int64_t* binS, const i
     }
     if (lo < outlength && outer_offsets[lo] == g.r) b0 = lo;
   }
+  if (!__any(b0 < outlength)) return;  // no bin starts at this wave's rows (wave-uniform)
   // columns at or past the group's longest row: S(r, c) is the count before the group
   for (int c = 0; c < (int)maxcount; c++) {
     const int64_t s = c < g.ml ? rows_before(g, c, chunkpre, ngroups, j, s_w)
@@ -570,8 +571,14 @@ __global__ __launch_bounds__(kBlock) void colrank_scatter_kernel(int64_t* out, c
                                                                  const int64_t* outer_offsets, int64_t outlength,
                                                                  int64_t row0, int64_t row1, int64_t maxcount,
                                                                  int64_t ngroups) {
+  // `missing` is written in row-major order: each wave stages its rows'
+  // values (when they fit and every element of the rows is written) and
+  // stores them coalesced
+  constexpr int kStage = PLACE ? 1 : 1024;
   __shared__ unsigned int s_w[kBlock / kWave][kFastMax];
   __shared__ int s_ml;
+  __shared__ int64_t s_out[kBlock / kWave][kStage];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int64_t j = blockIdx.x;
   const RowGroup g = row_group(offsets, row0, row1, maxcount, j, s_w, &s_ml);
   if (g.ml == 0) return;  // workgroup-uniform
@@ -581,13 +588,40 @@ __global__ __launch_bounds__(kBlock) void colrank_scatter_kernel(int64_t* out, c
     e0 = offsets[g.r];
     bstart = outer_offsets[bin];
   }
+  bool stage = false;
+  int64_t we0 = 0, we1 = 0;
+  if constexpr (!PLACE) {
+    const bool valid = g.r < row1;
+    int64_t rs = 0, re = 0;
+    if (valid) {
+      rs = offsets[g.r];
+      re = offsets[g.r + 1];
+    }
+    const uint64_t vb = __ballot(valid);
+    if (vb != 0) {
+      const int last = 63 - __builtin_clzll(vb);
+      we0 = __shfl(rs, 0, kWave);
+      we1 = __shfl(re, last, kWave);
+      stage = !__any(valid && (re - rs > maxcount || re < rs)) && we1 >= we0 && we1 - we0 <= kStage;
+    }
+  }
   for (int c = 0; c < g.ml; c++) {
     const int64_t s = rows_before(g, c, chunkpre, ngroups, j, s_w);
     if (c < g.len) {
       const int64_t rank = s - binS[bin * maxcount + c];
-      if (PLACE) out[prefix[bin * maxcount + c] + rank] = e0 + c;
-      else out[e0 + c] = (g.r - bstart) - rank;
+      if constexpr (PLACE) {
+        out[prefix[bin * maxcount + c] + rank] = e0 + c;
+      } else {
+        const int64_t v = (g.r - bstart) - rank;
+        if (stage) s_out[wave][e0 + c - we0] = v;
+        else out[e0 + c] = v;
+      }
     }
+  }
+  if constexpr (!PLACE) {
+    __syncthreads();
+    if (stage)
+      for (int64_t k = lane; k < we1 - we0; k += kWave) out[we0 + k] = s_out[wave][k];
   }
 }
 
