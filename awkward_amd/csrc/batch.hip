@@ -167,7 +167,7 @@ Error launch_batch_variant(int variant, const BatchReduceItem* it, int nitems, i
     case 3: return launch_batch_reduce<T, S, 128, 1, 16384>(it, nitems, tiles, counter, identity, choice, 3, q);
     case 4: return launch_batch_reduce<T, S, 256, 1, 65536>(it, nitems, tiles, counter, identity, choice, 4, q);
     case 5: return launch_batch_reduce<T, S, 256, 8, 8192>(it, nitems, tiles, counter, identity, choice, 5, q);
-    case 6: return launch_batch_reduce<T, S, 64, 1, 8192>(it, nitems, tiles, counter, identity, choice, 6, q);
+    case 6: return launch_batch_reduce<T, S, 64, 1, 6144>(it, nitems, tiles, counter, identity, choice, 6, q);
     case 7: return launch_batch_reduce<T, S, 128, 2, 16384>(it, nitems, tiles, counter, identity, choice, 7, q);
     case 8: return launch_batch_reduce<T, S, 64, 4, 8192>(it, nitems, tiles, counter, identity, choice, 8, q);
     case 9: return launch_batch_reduce<T, S, 128, 1, 8192>(it, nitems, tiles, counter, identity, choice, 9, q);

@@ -684,7 +684,7 @@ inline Error launch_reduce_queued(const Op& op, const Epi& epi, const typename O
 // LDS tiles let more workgroups -- and more loads -- be in flight per CU.
 //   0: 256 x 1, 32 KiB    1: 256 x 4, 8 KiB     2: 256 x 2, 16 KiB
 //   3: 128 x 1, 16 KiB    4: 256 x 1, 64 KiB    5: 256 x 8, 8 KiB
-//   6:  64 x 1,  8 KiB    7: 128 x 2, 16 KiB    8:  64 x 4,  8 KiB
+//   6:  64 x 1,  6 KiB    7: 128 x 2, 16 KiB    8:  64 x 4,  8 KiB
 //   9: 128 x 1,  8 KiB
 //  10: the long-list queue above (chosen for mean lengths of thousands; not
 //      one of the searched tiles: it synchronises once)
@@ -710,7 +710,7 @@ inline Error launch_reduce(const Op& op, const typename Op::In* fromptr, const i
     case 3: return launch_reduce_v<Op, 128, 1, 16384, Epi>(op, epi, fromptr, offsets, outlength);
     case 4: return launch_reduce_v<Op, 256, 1, 65536, Epi>(op, epi, fromptr, offsets, outlength);
     case 5: return launch_reduce_v<Op, 256, 8, 8192, Epi>(op, epi, fromptr, offsets, outlength);
-    case 6: return launch_reduce_v<Op, 64, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength);
+    case 6: return launch_reduce_v<Op, 64, 1, 6144, Epi>(op, epi, fromptr, offsets, outlength);
     case 7: return launch_reduce_v<Op, 128, 2, 16384, Epi>(op, epi, fromptr, offsets, outlength);
     case 8: return launch_reduce_v<Op, 64, 4, 8192, Epi>(op, epi, fromptr, offsets, outlength);
     case 9: return launch_reduce_v<Op, 128, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength);

@@ -26,7 +26,7 @@ VARIANTS = [
     (128, 1, 16384),
     (256, 1, 65536),
     (256, 8, 8192),
-    (64, 1, 8192),
+    (64, 1, 6144),
     (128, 2, 16384),
     (64, 4, 8192),
     (128, 1, 8192),
