@@ -194,26 +194,28 @@ struct ArgOp {
   using Part = ValIdx<T>;
   __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t gb) const {
     if (b >= e) return -1;
-    int64_t best = b;
-    T best_val = p[b];
-    int64_t i = b + 1;
-    for (; i + 4 <= e; i += 4) {
-      const T v[4] = {p[i], p[i + 1], p[i + 2], p[i + 3]};
+    // positions within the list in 32 bits (lists here hold < 2^31 elements)
+    const T* q = p + b;
+    const int n = (int)(e - b);
+    int best = 0;
+    T best_val = q[0];
+    int i = 1;
+    for (; i + 4 <= n; i += 4) {
+      const T v[4] = {q[i], q[i + 1], q[i + 2], q[i + 3]};
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (MAX ? (v[u] > best_val) : (v[u] < best_val)) {
-          best_val = v[u];
-          best = i + u;
-        }
-    }
-    for (; i < e; i++) {
-      const T v = p[i];
-      if (MAX ? (v > best_val) : (v < best_val)) {
-        best_val = v;
-        best = i;
+      for (int u = 0; u < 4; u++) {
+        const bool take = MAX ? (v[u] > best_val) : (v[u] < best_val);
+        best_val = take ? v[u] : best_val;
+        best = take ? i + u : best;
       }
     }
-    return gb + (best - b);
+    for (; i < n; i++) {
+      const T v = q[i];
+      const bool take = MAX ? (v > best_val) : (v < best_val);
+      best_val = take ? v : best_val;
+      best = take ? i : best;
+    }
+    return gb + best;
   }
   __device__ Part init() const { return Part{T(0), -1}; }
   __device__ Part elem(In v, int64_t gi) const { return Part{v, is_nan(v) ? -1 : gi}; }
