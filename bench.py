@@ -528,8 +528,9 @@ def main():
     ap.add_argument("--roofline-probe", action="store_true", help="every workload's kernels once, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each op is independent per "
-              "list; sharding by list ranges is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (each op is independent per list; a "
+              "sharded form would give each GPU whole lists and exchange only the k per-GPU totals that "
+              "offsets need)", file=sys.stderr)
         sys.exit(2)
 
     if args.traffic_probe:
