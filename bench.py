@@ -15,7 +15,7 @@ path with tuned reductions.  `roofline` is the HBM roofline of the fused
 form's kernel (algorithmic bytes / its average duration, from HIP events
 recorded around each replay on the stream it runs on).  `cpu_baseline` times the same
 pipeline through the same host layer on the single-threaded CPU oracle over a
-bounded sample of the events.  `config1` reports the segmented reductions of
+bounded sample of the events.  `segmented_reductions` reports the segmented reductions of
 the 1M-list Poisson(10) float64 layout, HBM-cold (a read-only 1 GiB scrub
 before each timed launch).
 
@@ -66,7 +66,7 @@ def pipeline(ak, ev):
     return ak.num(sel), ak.sum(sel.pt, axis=-1), ak.argmax(sel.pt, axis=-1)
 
 
-def run_config1(ak, backend, torch, reps=5):
+def run_segmented_reductions(ak, backend, torch, reps=5):
     """Segmented reductions on 1M Poisson(10) float64 lists, HBM-cold."""
     import numpy as np
 
@@ -123,7 +123,7 @@ def run_config1(ak, backend, torch, reps=5):
     ops["read_ceiling"] = (lambda: read(dev_x.data_ptr(), 8 * N, sink.data_ptr()), None, 8 * N)
     dispatch.reduce_sum(out_f, dev_x, dev_off, n, v)
     torch.cuda.synchronize()
-    ACCURACY["config1"] = (off, x, out_f.cpu().numpy())
+    ACCURACY["segmented_reductions"] = (off, x, out_f.cpu().numpy())
     for name, (key, args, nbytes) in list(ops.items()) + [("sum_best_dirty_scrub", ops["sum_best"])]:
         if args is None:
             call = key
@@ -150,7 +150,7 @@ def run_config1(ak, backend, torch, reps=5):
     return result
 
 
-def config3_step(ak, np, ev):
+def pairs_step(ak, np, ev):
     pairs = ak.combinations(ev, 2)
     p0, p1 = pairs["0"], pairs["1"]
     expr = 2 * p0.pt * p1.pt * (np.cosh(p0.eta - p1.eta) - np.cos(p0.phi - p1.phi))
@@ -158,7 +158,7 @@ def config3_step(ak, np, ev):
     return ak.max(expr, axis=-1), ak.argmax(expr, axis=-1), triples
 
 
-def run_config3(ak, backend, torch, n_events=20_000_000, steps=5):
+def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
     """Combinations n=2 and n=3 on 20M events (Poisson(4) clipped [0, 30],
     seed 3), the float32 pair expression, max/argmax over pairs per event."""
     import numpy as np
@@ -170,7 +170,7 @@ def run_config3(ak, backend, torch, n_events=20_000_000, steps=5):
     ev = ak.from_offsets(off, fields, backend)
     torch.cuda.synchronize()
     out = {"events": n_events, "particles": int(off[-1])}
-    out["plain"] = time_form(torch, lambda: config3_step(ak, np, ev), steps, 1, n_events)
+    out["plain"] = time_form(torch, lambda: pairs_step(ak, np, ev), steps, 1, n_events)
     pm = PairMax(ev)
     tri = Combinations(ev, 3)
     out["pairs"] = int(ak.num(ak.combinations(ev, 2)).layout.data.sum().item())
@@ -200,7 +200,7 @@ def run_config3(ak, backend, torch, n_events=20_000_000, steps=5):
 
 
 def make_batch(seed=4, narrays=256, min_exp=3.0, max_exp=5.0, zipf_cap=100_000):
-    """The config-4 generator (seed 4): per array a list count 10**U(min_exp,
+    """The batched workload's generator (seed 4): per array a list count 10**U(min_exp,
     max_exp), lengths Poisson(2) / Poisson(50) / Zipf(1.5) capped, 5% of the
     arrays all-empty, dtype int32 / int64 / float32 / float64, values
     integers(-1000, 1000) or standard normal."""
@@ -228,7 +228,7 @@ def make_batch(seed=4, narrays=256, min_exp=3.0, max_exp=5.0, zipf_cap=100_000):
     return out
 
 
-def run_config4(ak, torch, steps=3):
+def run_batched(ak, torch, steps=3):
     """Batched uneven workload: per array sum, min, argmin and argsort.
     plain = the registry path array by array; batched = the persistent
     work-queue kernels (autotuned tiles) replayed from one hipGraph."""
@@ -275,7 +275,7 @@ def run_config4(ak, torch, steps=3):
     return out
 
 
-def run_config5(torch, budget_s=40.0):
+def run_dispatch_regret(torch, budget_s=40.0):
     """Learned dispatch: the trained cost model's regret against the
     exhaustive sweep (every tile timed) on held-out arrays -- seeds 9000+,
     drawn on the device by the training generators -- beside the autotuner
@@ -349,9 +349,9 @@ def traffic_probe(args):
 def roofline_probe(args):
     """The program the per-kernel counter runs profile: the 1 GiB copy the
     counters are calibrated on, then every workload's kernels once, eagerly
-    (graph replays are not profiled per dispatch): config 1's reductions
-    (plain and tuned), the event pipeline (plain, tuned and fused), config
-    3's combinations (plain and fused) and a config-4 batch (sum/min/argmin
+    (graph replays are not profiled per dispatch): the 1M-list reductions
+    (plain and tuned), the event pipeline (plain, tuned and fused), the
+    20M-event combinations (plain and fused) and a 256-array batch (sum/min/argmin
     and argsort)."""
     import numpy as np
     import torch
@@ -388,7 +388,7 @@ def roofline_probe(args):
     off3, f3 = make_events(20_000_000, seed=3, lam=4, clip=30)
     f3 = {k: f3[k] for k in ("pt", "eta", "phi")}
     ev3 = ak.from_offsets(off3, f3, tuned)
-    config3_step(ak, np, ev3)
+    pairs_step(ak, np, ev3)
     PairMax(ev3).run()
     Combinations(ev3, 3).run()
     torch.cuda.synchronize()
@@ -398,7 +398,7 @@ def roofline_probe(args):
     BatchRunner(dev, variants={n: 0 for n in ("int32", "int64", "float32", "float64")}).run()
     torch.cuda.synchronize()
     del dev
-    # option types and non-local reductions on the config-1 layout: 10% of the
+    # option types and non-local reductions on the 1M-list layout: 10% of the
     # values missing (IndexedOptionArray), and the same lists reduced across
     # (axis=0: by position)
     from awkward_amd import contents as C
@@ -458,7 +458,7 @@ def _fsum_errors(off, values, sums, exact):
 
 def accuracy_report(ak, off, fields, gpu_sum, sample_events):
     """Float sums against math.fsum, for the GPU and the single-threaded CPU
-    oracle: config 1's float64 sums (all 1M lists) and the event pipeline's
+    oracle: the 1M-list float64 sums and the event pipeline's
     float32 sum(pt) over the kept particles (the first `sample_events`
     events).  Error = |sum - fsum| / sum(|x|) per list."""
     import math
@@ -469,12 +469,12 @@ def accuracy_report(ak, off, fields, gpu_sum, sample_events):
 
     be = OracleBackend()
     report = {"definition": "per list |sum - math.fsum| / sum(|x|); max and mean over lists"}
-    if "config1" in ACCURACY:
-        off1, x1, g1 = ACCURACY["config1"]
+    if "segmented_reductions" in ACCURACY:
+        off1, x1, g1 = ACCURACY["segmented_reductions"]
         exact = np.array([math.fsum(x1[a:b]) for a, b in zip(off1[:-1], off1[1:])])
         c1 = ak.sum(ak.from_offsets(off1, x1, be), axis=-1).layout
         cpu1 = np.asarray(c1.nplike.to_numpy(c1.data))
-        report["config1_sum_f64"] = {"lists": len(exact), "gpu": _fsum_errors(off1, x1, g1, exact),
+        report["segmented_sum_f64"] = {"lists": len(exact), "gpu": _fsum_errors(off1, x1, g1, exact),
                                      "cpu": _fsum_errors(off1, x1, cpu1, exact),
                                      "gpu_bits_equal_cpu": float(np.mean(g1.view(np.int64) == cpu1.view(np.int64)))}
     n = min(sample_events, len(off) - 1)
@@ -494,7 +494,7 @@ def accuracy_report(ak, off, fields, gpu_sum, sample_events):
     c2 = pipeline(ak, sub)[1].layout
     cpu2 = np.asarray(c2.nplike.to_numpy(c2.data))
     g2 = gpu_sum[:n]
-    report["config2_sum_pt_f32"] = {"events": n, "gpu": _fsum_errors(kept_off, kept_pt, g2, exact2),
+    report["pipeline_sum_pt_f32"] = {"events": n, "gpu": _fsum_errors(kept_off, kept_pt, g2, exact2),
                                     "cpu": _fsum_errors(kept_off, kept_pt, cpu2, exact2),
                                     "gpu_bits_equal_cpu": float(np.mean(g2.view(np.int32) == cpu2.view(np.int32)))}
     return report
@@ -520,10 +520,10 @@ def main():
     ap.add_argument("--events", type=int, default=50_000_000)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-config1", action="store_true")
-    ap.add_argument("--no-config3", action="store_true")
-    ap.add_argument("--no-config4", action="store_true")
-    ap.add_argument("--no-config5", action="store_true")
+    ap.add_argument("--no-reductions", action="store_true", help="skip the 1M-list segmented reductions")
+    ap.add_argument("--no-combinations", action="store_true", help="skip the 20M-event combinations")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-array batch")
+    ap.add_argument("--no-dispatch", action="store_true", help="skip the dispatcher regret sweep")
     ap.add_argument("--traffic-probe", action="store_true", help="only the program profiled by the counter runs")
     ap.add_argument("--roofline-probe", action="store_true", help="every workload's kernels once, for counter runs")
     args = ap.parse_args()
@@ -596,20 +596,20 @@ def main():
                 "kernel": "event_pipeline_kernel (awkward_hip_event_pipeline)", "kernel_ms": round(kms, 4),
                 "alg_bytes": alg_bytes}
 
-    config1 = None if args.no_config1 else run_config1(ak, tuned_be, torch)
+    reductions = None if args.no_reductions else run_segmented_reductions(ak, tuned_be, torch)
     del ev, ev_plain, pipe, graph
     torch.cuda.empty_cache()
-    config3 = None if args.no_config3 else run_config3(ak, tuned_be, torch)
-    if config3:
-        log(f"config3: plain {config3['plain']['events_per_s']:.4g} ev/s, fused {config3['fused_graph']['events_per_s']:.4g} ev/s")
+    combos = None if args.no_combinations else run_combinations(ak, tuned_be, torch)
+    if combos:
+        log(f"combinations: plain {combos['plain']['events_per_s']:.4g} ev/s, fused {combos['fused_graph']['events_per_s']:.4g} ev/s")
     torch.cuda.empty_cache()
-    config4 = None if args.no_config4 else run_config4(ak, torch)
-    if config4:
-        log(f"config4: plain {config4['plain']['lists_per_s']:.4g} lists/s, batched {config4['batched_graph']['lists_per_s']:.4g} lists/s")
-    config5 = None if args.no_config5 else run_config5(torch)
-    if config5:
-        log(f"config5: learned regret {config5['learned']['aggregate']}, table {config5['table']['aggregate']}, "
-            f"heuristic {config5['heuristic']['aggregate']} over {config5['rows']} held-out rows")
+    batched = None if args.no_batched else run_batched(ak, torch)
+    if batched:
+        log(f"batched: plain {batched['plain']['lists_per_s']:.4g} lists/s, batched {batched['batched_graph']['lists_per_s']:.4g} lists/s")
+    regret = None if args.no_dispatch else run_dispatch_regret(torch)
+    if regret:
+        log(f"dispatch: learned regret {regret['learned']['aggregate']}, table {regret['table']['aggregate']}, "
+            f"heuristic {regret['heuristic']['aggregate']} over {regret['rows']} held-out rows")
     cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
     accuracy = None if args.no_cpu else accuracy_report(ak, off, fields, gpu_sum_pt, args.cpu_sample)
     if accuracy:
@@ -636,10 +636,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
-        "config1": config1,
-        "config3": config3,
-        "config4": config4,
-        "config5": config5,
+        "segmented_reductions": reductions,
+        "combinations": combos,
+        "batched": batched,
+        "dispatch_regret": regret,
     }
     print(json.dumps(line), flush=True)
 

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def make_batch(seed, narrays=24, max_exp=4.3):
-    """A small batch with the config-4 mix: list counts log-uniform, list
+    """A small batch with the batched workload's mix: list counts log-uniform, list
     lengths Poisson(2) / Poisson(50) / Zipf(1.5) capped, some all-empty
     arrays, int32 / int64 / float32 / float64 values."""
     rng = np.random.default_rng(seed)

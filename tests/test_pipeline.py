@@ -53,7 +53,7 @@ def flat(array):
     raise TypeError(layout)
 
 
-def config2_pipeline(ev):
+def event_pipeline_steps(ev):
     mask = (ev.pt > 25) & (abs(ev.eta) < 2.4)
     sel = ev[mask][:, 1:]
     return {
@@ -64,17 +64,17 @@ def config2_pipeline(ev):
     }
 
 
-def test_config2_pipeline(backends):
+def test_event_pipeline(backends):
     cpu, hip = backends
     off, fields = events(2, 20000, 3, 20)
-    out_c = config2_pipeline(ak.from_offsets(off, fields, cpu))
-    out_h = config2_pipeline(ak.from_offsets(off, fields, hip))
+    out_c = event_pipeline_steps(ak.from_offsets(off, fields, cpu))
+    out_h = event_pipeline_steps(ak.from_offsets(off, fields, hip))
     for k in out_c:
         for a, b in zip(flat(out_c[k]), flat(out_h[k])):
             assert same_bits(a, b), k
 
 
-def config3_pipeline(ev, n):
+def pair_pipeline_steps(ev, n):
     pairs = ak.combinations(ev, n)
     p0, p1 = pairs["0"], pairs["1"]
     expr = 2 * p0.pt * p1.pt * (np.cosh(p0.eta - p1.eta) - np.cos(p0.phi - p1.phi))
@@ -82,11 +82,11 @@ def config3_pipeline(ev, n):
 
 
 @pytest.mark.parametrize("n", [2, 3])
-def test_config3_pipeline(backends, n):
+def test_pair_pipeline(backends, n):
     cpu, hip = backends
     off, fields = events(3, 5000, 4, 30)
-    e_c, m_c, a_c = config3_pipeline(ak.from_offsets(off, fields, cpu), n)
-    e_h, m_h, a_h = config3_pipeline(ak.from_offsets(off, fields, hip), n)
+    e_c, m_c, a_c = pair_pipeline_steps(ak.from_offsets(off, fields, cpu), n)
+    e_h, m_h, a_h = pair_pipeline_steps(ak.from_offsets(off, fields, hip), n)
     oc, vc = flat(e_c)
     oh, vh = flat(e_h)
     assert same_bits(oc, oh)
@@ -100,7 +100,7 @@ def test_config3_pipeline(backends, n):
         assert same_bits(a, b)
 
 
-def test_config1_reductions(backends):
+def test_segmented_reductions_1m_layout(backends):
     cpu, hip = backends
     rng = np.random.default_rng(1)
     counts = rng.poisson(10, 20000)
