@@ -119,3 +119,16 @@ def test_axis_none_default(make):
     assert ak.sum(nested) == 8.0 and ak.argmax(nested) == 2
     empty = ak.from_list([[], []], be, dtype=np.float64)
     assert ak.max(empty) is None and ak.sum(empty) == 0.0 and ak.count(empty) == 0
+
+
+def test_completely_flatten_drops_missing_values():
+    """The numbers axis=None reduces: every list level flattened in order,
+    missing values (option types at any level) dropped."""
+    from awkward_amd import contents as C
+
+    be = oracle_backend()
+    a = ak.from_list([[[1, None], []], None, [[2], [None, 3]]], be, dtype=np.int64)
+    flat = C.completely_flatten(a.layout)
+    assert flat.to_list() == [1, 2, 3]
+    with pytest.raises(TypeError):
+        C.completely_flatten(ak.from_offsets(np.array([0, 1]), {"x": np.array([1.0])}, be).layout)
