@@ -579,8 +579,13 @@ __global__ __launch_bounds__(kBlock) void long_corank_kernel(const LongChunk* __
 // elements [c * kCap, (c + 1) * kCap) of the merged pair of runs holding
 // them, between the co-ranks long_corank_kernel found; the pass that
 // completes a segment writes its output.
+// Merge passes use workgroups of 512 threads, 8 outputs each: twice the
+// waves per CU of 256-thread blocks at the same LDS (the merge is bound by
+// LDS latency): 40.5 -> 32.8 ms on the 256-array batch.
+constexpr int kMergeBlock = 512;
+
 template <typename T, bool ARG>
-__global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __restrict__ chunks,
+__global__ __launch_bounds__(kMergeBlock) void long_merge_kernel(const LongChunk* __restrict__ chunks,
                                                             const unsigned long long* __restrict__ count,
                                                             const unsigned long long* __restrict__ maxlen,
                                                             int64_t capacity, int64_t w,
@@ -590,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
                                                             const int64_t* __restrict__ coranks) {
   using K = KeyOf<T>;
   if ((int64_t)*maxlen <= w) return;  // every queued segment is already one run
-  constexpr int PER = kCap / kBlock;   // outputs per thread
+  constexpr int PER = kCap / kMergeBlock;   // outputs per thread
   __shared__ uint64_t s_raw[lds_pair_words<K>(kCap)];
   const LdsPairs<K> sp = lds_pairs<K>(s_raw, kCap);
   int64_t nc = (int64_t)*count;
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
     const int64_t y0 = s_diag[0] - x0, y1 = s_diag[1] - x1;
     const int nal = (int)(x1 - x0), nbl = (int)(y1 - y0);
     const int n = nal + nbl;
-    for (int i = tid; i < n; i += kBlock) {
+    for (int i = tid; i < n; i += kMergeBlock) {
       if (i < nal) sp.put(i, ka[x0 + i], ia[x0 + i]);
       else sp.put(i, kb[y0 + i - nal], ib[y0 + i - nal]);
     }
@@ -658,9 +663,9 @@ __global__ __launch_bounds__(kBlock) void long_merge_kernel(const LongChunk* __r
       if (o0 + u < o1) sp.put(o0 + u, rk[u], ri[u]);
     __syncthreads();
     if (2 * w >= ch.len) {
-      for (int i = tid; i < n; i += kBlock) store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, d0 + i, sp.idx(i));
+      for (int i = tid; i < n; i += kMergeBlock) store_sorted<T, ARG>(ch.out, (const T*)ch.from, ch.a, d0 + i, sp.idx(i));
     } else {
-      for (int i = tid; i < n; i += kBlock) {
+      for (int i = tid; i < n; i += kMergeBlock) {
         K kk;
         uint32_t xx;
         sp.get(i, kk, xx);
@@ -694,7 +699,7 @@ inline hipError_t launch_long_sort(const LongQueue& q, int64_t max_len, bool asc
   for (int64_t w = kCap; w < max_len; w <<= 1) {
     long_corank_kernel<K><<<(unsigned)cgrid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
                                                              q.coranks);
-    long_merge_kernel<T, ARG><<<(unsigned)grid, kBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
+    long_merge_kernel<T, ARG><<<(unsigned)grid, kMergeBlock, 0, st>>>(q.chunks, q.count, q.maxlen, q.capacity, w, sk, si,
                                                                  dk, di, q.coranks);
     K* tk = sk;
     sk = dk;
