@@ -174,9 +174,12 @@ __device__ inline bool keep_particle(const TermRT* rt, int64_t i, int64_t local)
 // coalesced loads -- every float32 field's loads issued together -- and
 // staged (reduced value and predicate flag, one 8-byte word per particle,
 // so each particle costs one LDS read in the per-event loops) in an LDS stage sized at launch
-// (dynamic LDS), to keep many loads in flight per CU.
+// (dynamic LDS), to keep many loads in flight per CU.  Eight waves per SIMD
+// are asked for: left to itself the compiler keeps ~106 SGPRs (the resolved
+// terms and field pointers), which holds a SIMD to seven of these one-wave,
+// latency-bound workgroups; the extra scalars spill to VGPR lanes instead.
 template <int BLOCK, int LPT, int NT>
-__global__ __launch_bounds__(BLOCK) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
+__global__ __launch_bounds__(BLOCK, 8) void event_pipeline_kernel(const int64_t* __restrict__ offsets, int64_t n,
                                                                Fields fields, Predicate pred, Stage st,
                                                                int64_t start, int64_t stop, int64_t step, int red,
                                                                PipelineOut out) {
