@@ -4,8 +4,9 @@ jagged-array kernel layer.
 The kernels are hand-written HIP for gfx950 in ``libawkward_hip.so``
 (``awkward_amd/csrc``), served under the reference's kernel names through
 ``HipBackend`` (``backend[(name, *dtypes)](*args)``), and driven by a host
-layer that issues the reference's call sequences for ``sum``, ``max``,
-``min``, ``argmax``, ``argmin``, ``count``, ``num``, ``combinations``,
+layer that issues the reference's call sequences for ``sum``, ``prod``,
+``any``, ``all``, ``max``, ``min``, ``argmax``, ``argmin``, ``count``,
+``num``, ``combinations``,
 ``argcombinations``,
 jagged and range ``__getitem__``, ``sort`` and ``argsort`` at axis=-1.
 Buffers enter and leave without copies through DLPack (``from_dlpack``)
@@ -14,6 +15,8 @@ and the Form/length/container triple (``to_buffers`` / ``from_buffers``).
 
 from awkward_amd.highlevel import (  # noqa: F401
     Array,
+    all,
+    any,
     argcombinations,
     argmax,
     argmin,
@@ -26,6 +29,7 @@ from awkward_amd.highlevel import (  # noqa: F401
     max,
     min,
     num,
+    prod,
     sort,
     sum,
     to_backend,

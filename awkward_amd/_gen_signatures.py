@@ -34,6 +34,10 @@ FAMILIES = [
     "awkward_reduce_sum_int32_bool_64",
     "awkward_reduce_max",
     "awkward_reduce_min",
+    # ak.prod, ak.any / ak.all, and ak.max / ak.min of booleans (_reducers.py:676-758, 760-858, 899-913, 1046-1062)
+    "awkward_reduce_prod",
+    "awkward_reduce_sum_bool",
+    "awkward_reduce_prod_bool",
     "awkward_reduce_argmax",
     "awkward_reduce_argmin",
     "awkward_reduce_count_64",

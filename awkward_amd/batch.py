@@ -181,6 +181,7 @@ class BatchRunner:
         lib_.awkward_hip_reserve_scratch.argtypes = [ctypes.c_int64]
         lib_.awkward_hip_reserve_queue.argtypes = [ctypes.c_int64]
         sort_bytes = 24 * need + (need // 65 + 2) * 48 + (need // 4096 + need // 1025 + 4) * 64 + 8192
+        set_stream()  # the pools belong to the current stream
         if lib_.awkward_hip_reserve_scratch(int(sort_bytes)):
             raise MemoryError("cannot reserve the batch runner's scratch memory")
         if lib_.awkward_hip_reserve_queue(int((need // 8192 + need // 4097 + 4) * 96 + 8192)):

@@ -133,6 +133,13 @@ class NumpyArray(Content):
 class RecordArray(Content):
     @property
     def depth(self):
+        """The fields' list depth (their minimum, as purelist_depth takes it)."""
+        return builtins_min((c.depth for c in self.contents), default=1)
+
+    def _reduce_axis(self, reducer, negaxis, starts, shifts, offsets, outlength, mask, keepdims):
+        raise NotImplementedError("reductions of records are not on the hip path (reduce a field)")
+
+    def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
         raise NotImplementedError("reductions of records are not on the hip path (reduce a field)")
 
     def __init__(self, contents, fields, length, backend):

@@ -1,8 +1,8 @@
 // common.h -- shared pieces of libawkward_hip.so (gfx950 / CDNA4).
 //
 // * struct Error and kSliceNone, as in awkward-cpp/include/awkward/common.h:39-78
-// * the runtime state: current stream, error mode, device error word and a
-//   grow-only scratch pool per device
+// * the runtime state: current stream, error mode, and per (device, stream)
+//   a device error word and grow-only scratch pools
 // * small device helpers (wave reductions, launch geometry)
 #pragma once
 
@@ -32,17 +32,20 @@ inline Error failure(const char* msg, int64_t identity, int64_t attempt, const c
 }
 
 // ---- runtime (runtime.hip) ------------------------------------------------
+// State is kept per (device, stream): the stream set by awkward_hip_set_stream
+// on the calling thread, on the current device.
 hipStream_t stream();
 bool deferred_errors();
-// Device error word of the current device (uint64, kNoError when clear).
+// Device error word of the current stream (uint64, kNoError when clear);
+// nullptr if the stream's state could not be allocated.
 uint64_t* error_word();
 // Next invocation tag for an error-capable launch (fits 16 bits).
 uint32_t next_invocation(const char* kernel, const char* const* messages);
 // Synchronous mode: wait, read and clear the error word.  Returns kNoError
 // or the packed key (invocation << 48 | identity << 8 | code).
 uint64_t take_error();
-// Scratch memory of at least `bytes` on the current device, valid until the
-// next call of scratch() on this device (stream-ordered reuse).
+// Scratch memory of at least `bytes` of the current stream, valid until the
+// next call of scratch() on this stream (stream-ordered reuse).
 void* scratch(size_t bytes);
 // A second such pool, for work queues that live across the kernels of one
 // entry point while its caller may hold scratch() memory.
@@ -66,6 +69,16 @@ __device__ inline void record_error(uint64_t* word, uint32_t invocation, int64_t
   const uint64_t key = (uint64_t(invocation & 0xffff) << 48) | (id << 8) | uint64_t(code & 0xff);
   atomicMin((unsigned long long*)word, (unsigned long long)key);
 }
+
+// Declares `var` = the current stream's error word, or fails the entry point
+// when the stream's state cannot be allocated (no device memory, or a first
+// use of the stream under graph capture).
+#define AKHIP_ERROR_WORD(var)                                                                          \
+  uint64_t* var = ::akhip::error_word();                                                               \
+  if (var == nullptr)                                                                                  \
+  return ::akhip::failure("no device state for this stream (out of device memory, or its first use "   \
+                          "under graph capture)",                                                      \
+                          ::akhip::kSliceNone, ::akhip::kSliceNone, "awkward_amd/csrc/runtime.hip")
 
 #define AKHIP_LAUNCH_CHECK(what)                                  \
   do {                                                            \

@@ -192,10 +192,10 @@ Error awkward_hip_bool_to_index(int64_t* toindex, int64_t* tooffsets, const bool
   void* base = scratch(scan_scratch_bytes(ntiles));
   if (!base) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
   const ScanScratch s = scan_scratch_at(base);
+  AKHIP_ERROR_WORD(ew);
   hipStream_t st = stream();
   scan_prep_kernel<0><<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(s, ntiles);
-  bool_to_index_kernel<<<(unsigned)ntiles, kB2IBlock, 0, st>>>(toindex, tooffsets, mask, offsets, n, s, error_word(),
-                                                               0);
+  bool_to_index_kernel<<<(unsigned)ntiles, kB2IBlock, 0, st>>>(toindex, tooffsets, mask, offsets, n, s, ew, 0);
   AKHIP_LAUNCH_CHECK("bool_to_index_kernel");
   return success();
 }

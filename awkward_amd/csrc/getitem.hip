@@ -355,9 +355,10 @@ Error jagged_apply(int64_t* tooffsets, int64_t* tocarry, const int64_t* slicesta
     AKHIP_LAUNCH_CHECK("fill_i64");
     return success();
   }
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_ListArray_getitem_jagged_apply", kJaggedApplyMsgs);
   JaggedApplyBody<C> body{tooffsets, tocarry, slicestarts, slicestops, sliceindex, sliceinnerlen,
-                          fromstarts, fromstops, contentlen, sliceouterlen, error_word(), inv};
+                          fromstarts, fromstops, contentlen, sliceouterlen, ew, inv};
   Error e = run_scan(body, sliceouterlen, inv);
   if (e.str) return e;
   return finish_errors(kJaggedApplyMsgs, kFile, [&](int64_t i, int code) -> int64_t {
@@ -381,8 +382,9 @@ Error jagged_descend(int64_t* tooffsets, const int64_t* slicestarts, const int64
     AKHIP_LAUNCH_CHECK("fill_i64");
     return success();
   }
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_ListArray_getitem_jagged_descend", kDescendMsgs);
-  DescendBody<C> body{tooffsets, slicestarts, slicestops, fromstarts, fromstops, error_word(), inv};
+  DescendBody<C> body{tooffsets, slicestarts, slicestops, fromstarts, fromstops, ew, inv};
   Error e = run_scan(body, sliceouterlen, inv);
   if (e.str) return e;
   return finish_errors(kDescendMsgs, kFile, no_attempt);
@@ -416,9 +418,10 @@ Error next_range_counts(int64_t* total, const C* fromoffsets, int64_t lenstarts)
 template <typename C>
 Error next_at(int64_t* tocarry, const C* fromstarts, const C* fromstops, int64_t lenstarts, int64_t at) {
   if (lenstarts <= 0) return success();
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_ListArray_getitem_next_at", kIndexMsgs);
   next_at_kernel<C><<<grid_for(lenstarts, 256), 256, 0, stream()>>>(tocarry, fromstarts, fromstops, lenstarts, at,
-                                                                    error_word(), inv);
+                                                                    ew, inv);
   AKHIP_LAUNCH_CHECK("next_at_kernel");
   return finish_errors(kIndexMsgs, kFile, [&](int64_t, int) -> int64_t { return at; });
 }
@@ -430,8 +433,9 @@ Error compact_offsets(int64_t* tooffsets, const C* fromstarts, const C* fromstop
     AKHIP_LAUNCH_CHECK("fill_i64");
     return success();
   }
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_ListArray_compact_offsets", kCompactMsgs);
-  CompactBody<C> body{tooffsets, fromstarts, fromstops, error_word(), inv};
+  CompactBody<C> body{tooffsets, fromstarts, fromstops, ew, inv};
   Error e = run_scan(body, length, inv);
   if (e.str) return e;
   return finish_errors(kCompactMsgs, kFile, no_attempt);
@@ -440,9 +444,10 @@ Error compact_offsets(int64_t* tooffsets, const C* fromstarts, const C* fromstop
 template <typename C>
 Error indexed_nextcarry(int64_t* tocarry, const C* fromindex, int64_t lenindex, int64_t lencontent) {
   if (lenindex <= 0) return success();
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_IndexedArray_getitem_nextcarry", kIndexMsgs);
   nextcarry_kernel<C><<<grid_for(lenindex, 256), 256, 0, stream()>>>(tocarry, fromindex, lenindex, lencontent,
-                                                                     error_word(), inv);
+                                                                     ew, inv);
   AKHIP_LAUNCH_CHECK("nextcarry_kernel");
   return finish_errors(kIndexMsgs, kFile,
                        [&](int64_t i, int) -> int64_t { return (int64_t)read_device(fromindex, i); });

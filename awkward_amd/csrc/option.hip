@@ -235,8 +235,9 @@ template <typename C>
 Error indexed_nextcarry_outindex(int64_t* tocarry, C* toindex, const C* fromindex, int64_t lenindex,
                                  int64_t lencontent) {
   if (lenindex <= 0) return success();
+  AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_IndexedArray_getitem_nextcarry_outindex", kIndexMsgs);
-  IndexedOutindexBody<C> body{tocarry, toindex, fromindex, lencontent, error_word(), inv};
+  IndexedOutindexBody<C> body{tocarry, toindex, fromindex, lencontent, ew, inv};
   Error e = compact(body, lenindex, inv);
   if (e.str) return e;
   return finish_errors(kIndexMsgs, kFile,

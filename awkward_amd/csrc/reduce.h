@@ -129,6 +129,63 @@ struct SumBoolOp {  // awkward_reduce_sum_{int64,int32}_bool_64: count of true
   __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_any(p, m); }
 };
 
+// Integer products wrap (two's complement) like the CPU kernels' native multiplies.
+template <typename OUT>
+__device__ inline OUT wrap_mul(OUT a, OUT b) {
+  if constexpr (std::is_floating_point_v<OUT>) {
+    return a * b;
+  } else {
+    using U = std::make_unsigned_t<OUT>;
+    return (OUT)((U)a * (U)b);
+  }
+}
+
+// awkward_reduce_prod: acc = 1; acc *= (OUT)x in the CPU loop's order.
+template <typename OUT, typename IN>
+struct ProdOp {
+  using In = IN;
+  using Out = OUT;
+  using Part = OUT;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    OUT acc = OUT(1);
+    int64_t i = b;
+    for (; i + 4 <= e; i += 4) {
+      const In v0 = p[i], v1 = p[i + 1], v2 = p[i + 2], v3 = p[i + 3];
+      acc = wrap_mul<OUT>(acc, (OUT)v0);
+      acc = wrap_mul<OUT>(acc, (OUT)v1);
+      acc = wrap_mul<OUT>(acc, (OUT)v2);
+      acc = wrap_mul<OUT>(acc, (OUT)v3);
+    }
+    for (; i < e; i++) acc = wrap_mul<OUT>(acc, (OUT)p[i]);
+    return acc;
+  }
+  __device__ Part init() const { return OUT(1); }
+  __device__ Part elem(In v, int64_t) const { return (OUT)v; }
+  __device__ Part combine(Part a, Part b) const { return wrap_mul<OUT>(a, b); }
+  __device__ Out finalize(Part p, const In*, int64_t, int64_t) const { return p; }
+  __device__ Part shfl_xor(Part p, int m) const { return shfl_xor_any(p, m); }
+};
+
+// awkward_reduce_sum_bool (ANY) / awkward_reduce_prod_bool (!ANY): whether
+// any / every element is nonzero (x != 0, so NaN is nonzero and -0.0 zero).
+// Order-free, so every path gives the CPU kernel's bits.
+template <typename IN, bool ANY>
+struct AnyAllOp {
+  using In = IN;
+  using Out = bool;
+  using Part = int;
+  __device__ Out sequential(const In* p, int64_t b, int64_t e, int64_t) const {
+    int r = ANY ? 0 : 1;
+    for (int64_t i = b; i < e; i++) r = ANY ? (r | (p[i] != In(0))) : (r & (p[i] != In(0)));
+    return r != 0;
+  }
+  __device__ Part init() const { return ANY ? 0 : 1; }
+  __device__ Part elem(In v, int64_t) const { return v != In(0) ? 1 : 0; }
+  __device__ Part combine(Part a, Part b) const { return ANY ? (a | b) : (a & b); }
+  __device__ Out finalize(Part p, const In*, int64_t, int64_t) const { return p != 0; }
+  __device__ Part shfl_xor(Part p, int m) const { return __shfl_xor(p, m, kWave); }
+};
+
 template <typename T>
 struct ValIdx {
   T val;

@@ -1,14 +1,40 @@
-// runtime.hip -- stream, error word and scratch pool of libawkward_hip.so.
+// runtime.hip -- streams, error words and scratch pools of libawkward_hip.so.
 //
-// The reference CUDA backend keeps one error word per stream context and
-// raises lazily (src/awkward/_connect/cuda/__init__.py:28-40, 257-294).  Here
-// each device has one uint64 error word: error-capable kernels atomicMin a
-// key (invocation << 48 | failing list << 8 | code), so the earliest failure
-// in loop order of the earliest invocation wins, as in the CPU kernels.
+// The reference CUDA backend keeps its error state per stream
+// (src/awkward/_connect/cuda/__init__.py:28 cuda_streamptr_to_contexts, and
+// :257-294 for the lazy check).  Here every (device, stream) pair that calls
+// the library owns a StreamState:
+//
+//   * a uint64 device error word: error-capable kernels atomicMin a key
+//     (invocation << 48 | failing list << 8 | code), so the earliest failure
+//     in loop order of the earliest invocation wins, as in the CPU kernels;
+//     only kernels of this stream write it, and it is read and cleared on
+//     this stream, so streams never see each other's errors;
+//   * the invocation table of deferred mode (tag -> kernel and messages),
+//     bounded: at 65,536 entries the word is read once (one stream
+//     synchronisation), a recorded error is kept for the next poll, and the
+//     table restarts, so tags never alias;
+//   * two grow-only device pools (scratch, and work queues that must live
+//     across the kernels of one entry point).  A pool that grows retires its
+//     old buffer instead of freeing it: kernels in flight and captured
+//     hipGraphs may still use it, so a retired buffer stays allocated until
+//     the process ends (pools double, so the retired buffers of a pool add
+//     up to less than its current size).  Nothing on the call path
+//     synchronises the device or frees memory.
+//
+// Scratch is stream-ordered: the kernels of one stream use its pools one
+// after another, so two streams never share scratch.  A hipGraph captured on
+// a stream uses that stream's pools; replaying it concurrently with other
+// work of the same stream would race (replay it on its own stream, or
+// serialise).  The first call on a stream allocates its state (an 8-byte
+// error word and its pinned mirror): make it before capturing on that
+// stream (awkward_hip_reserve_scratch does).
 
-#include <mutex>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -17,27 +43,36 @@ namespace akhip {
 namespace {
 
 constexpr int kMaxDevices = 64;
+constexpr size_t kMaxInvocations = 65536;  // the 16-bit tag of pack_error
 
-struct DeviceState {
-  uint64_t* error_word = nullptr;  // device
-  uint64_t* host_word = nullptr;   // pinned host mirror
-  void* scratch = nullptr;
-  size_t scratch_bytes = 0;
-  void* queue = nullptr;  // second pool: work queues of multi-kernel sequences
-  size_t queue_bytes = 0;
-  int cus = 0;
+struct Pool {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  std::vector<void*> retired;
 };
-
-DeviceState g_dev[kMaxDevices];
-std::mutex g_mutex;
-thread_local hipStream_t t_stream = nullptr;
-bool g_deferred = false;
 
 struct Invocation {
   const char* kernel;
   const char* const* messages;
 };
-std::vector<Invocation> g_invocations;  // index = invocation tag (deferred mode)
+
+struct StreamState {
+  uint64_t* error_word = nullptr;  // device
+  uint64_t* host_word = nullptr;   // pinned host mirror
+  Pool scratch;
+  Pool queue;
+  std::vector<Invocation> invocations;  // deferred mode: index = invocation tag
+  Error pending{nullptr, nullptr, kSliceNone, kSliceNone};  // an error found when the table wrapped
+};
+
+int g_cus[kMaxDevices];
+std::mutex g_mutex;
+std::map<std::pair<int, hipStream_t>, StreamState*> g_states;
+thread_local hipStream_t t_stream = nullptr;
+thread_local StreamState* t_cached = nullptr;
+thread_local hipStream_t t_cached_stream = nullptr;
+thread_local int t_cached_device = -1;
+bool g_deferred = false;
 
 int current_device() {
   int d = 0;
@@ -47,91 +82,127 @@ int current_device() {
 
 __global__ void reset_word(uint64_t* w) { *w = kNoError; }
 
-DeviceState& state() {
+// The state of (current device, current stream); nullptr if its error word
+// cannot be allocated.
+StreamState* state() {
   const int d = current_device();
-  DeviceState& s = g_dev[d];
-  if (s.error_word == nullptr) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    if (s.error_word == nullptr) {
-      uint64_t* w = nullptr;
-      if (hipMalloc(&w, sizeof(uint64_t)) == hipSuccess) {
-        (void)hipMemset(w, 0xff, sizeof(uint64_t));
-        (void)hipHostMalloc((void**)&s.host_word, sizeof(uint64_t), hipHostMallocDefault);
-        (void)hipDeviceSynchronize();
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, d) == hipSuccess) s.cus = prop.multiProcessorCount;
-        s.error_word = w;
-      }
+  if (t_cached != nullptr && t_cached_device == d && t_cached_stream == t_stream) return t_cached;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  StreamState*& s = g_states[std::make_pair(d, t_stream)];
+  if (s == nullptr) {
+    uint64_t* w = nullptr;
+    uint64_t* h = nullptr;
+    if (hipMalloc(&w, sizeof(uint64_t)) != hipSuccess) return nullptr;
+    if (hipHostMalloc((void**)&h, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+      (void)hipFree(w);
+      return nullptr;
     }
+    reset_word<<<1, 1, 0, t_stream>>>(w);
+    if (hipGetLastError() != hipSuccess) return nullptr;
+    if (d >= 0 && d < kMaxDevices && g_cus[d] == 0) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, d) == hipSuccess) g_cus[d] = prop.multiProcessorCount;
+    }
+    s = new StreamState();
+    s->error_word = w;
+    s->host_word = h;
   }
+  t_cached = s;
+  t_cached_stream = t_stream;
+  t_cached_device = d;
   return s;
+}
+
+// Reads and clears the stream's error word (synchronises the stream).
+uint64_t read_and_clear(StreamState* s) {
+  hipStream_t st = t_stream;
+  if (hipMemcpyAsync(s->host_word, s->error_word, sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return kNoError;
+  if (hipStreamSynchronize(st) != hipSuccess) return kNoError;
+  const uint64_t key = *s->host_word;
+  if (key != kNoError) reset_word<<<1, 1, 0, st>>>(s->error_word);
+  return key;
+}
+
+Error describe(const StreamState* s, uint64_t key) {
+  const uint32_t inv = (uint32_t)(key >> 48);
+  const int code = error_code(key);
+  if (inv < s->invocations.size() && s->invocations[inv].messages != nullptr)
+    return Error{s->invocations[inv].messages[code], s->invocations[inv].kernel, error_identity(key), kSliceNone};
+  return Error{"kernel error (unknown invocation)", "awkward_amd/csrc/runtime.hip", error_identity(key), kSliceNone};
+}
+
+// Grow-only device buffer: at least `bytes`, contents not preserved on
+// growth; the old buffer is retired, not freed.
+void* grow(Pool* pool, size_t bytes) {
+  if (bytes < 256) bytes = 256;
+  if (pool->bytes < bytes) {
+    size_t want = pool->bytes ? pool->bytes : (size_t)1 << 20;
+    while (want < bytes) want *= 2;
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) return nullptr;
+    if (pool->buf) pool->retired.push_back(pool->buf);
+    pool->buf = p;
+    pool->bytes = want;
+  }
+  return pool->buf;
 }
 
 }  // namespace
 
 hipStream_t stream() { return t_stream; }
 bool deferred_errors() { return g_deferred; }
-uint64_t* error_word() { return state().error_word; }
+
+uint64_t* error_word() {
+  StreamState* s = state();
+  return s ? s->error_word : nullptr;
+}
 
 int cu_count() {
-  int c = state().cus;
+  const int d = current_device();
+  int c = (d >= 0 && d < kMaxDevices) ? g_cus[d] : 0;
+  if (c == 0) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess) c = prop.multiProcessorCount;
+    if (d >= 0 && d < kMaxDevices) g_cus[d] = c;
+  }
   return c > 0 ? c : 256;
 }
 
 uint32_t next_invocation(const char* kernel, const char* const* messages) {
+  StreamState* s = state();
+  if (s == nullptr) return 0;
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!g_deferred) {
-    g_invocations.clear();
+    s->invocations.clear();
+  } else if (s->invocations.size() >= kMaxInvocations) {
+    // the tag space is used up: settle the errors recorded so far
+    const uint64_t key = read_and_clear(s);
+    if (key != kNoError && s->pending.str == nullptr) s->pending = describe(s, key);
+    s->invocations.clear();
   }
-  g_invocations.push_back(Invocation{kernel, messages});
-  return (uint32_t)((g_invocations.size() - 1) & 0xffff);
+  s->invocations.push_back(Invocation{kernel, messages});
+  return (uint32_t)((s->invocations.size() - 1) & 0xffff);
 }
 
 uint64_t take_error() {
-  DeviceState& s = state();
-  hipStream_t st = t_stream;
-  if (hipMemcpyAsync(s.host_word, s.error_word, sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess)
-    return kNoError;
-  if (hipStreamSynchronize(st) != hipSuccess) return kNoError;
-  const uint64_t key = *s.host_word;
-  if (key != kNoError) {
-    reset_word<<<1, 1, 0, st>>>(s.error_word);
-    (void)hipStreamSynchronize(st);
-  }
-  return key;
-}
-
-// Grow-only device buffer: at least `bytes`, contents not preserved on growth.
-void* grow(void** buf, size_t* have, size_t bytes) {
-  if (bytes < 256) bytes = 256;
-  if (*have < bytes) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    if (*have < bytes) {
-      size_t want = *have ? *have : (size_t)1 << 20;
-      while (want < bytes) want *= 2;
-      if (*buf) {
-        (void)hipDeviceSynchronize();  // in-flight kernels may still use it
-        (void)hipFree(*buf);
-        *buf = nullptr;
-        *have = 0;
-      }
-      void* p = nullptr;
-      if (hipMalloc(&p, want) != hipSuccess) return nullptr;
-      *buf = p;
-      *have = want;
-    }
-  }
-  return *buf;
+  StreamState* s = state();
+  if (s == nullptr) return kNoError;
+  return read_and_clear(s);
 }
 
 void* scratch(size_t bytes) {
-  DeviceState& s = state();
-  return grow(&s.scratch, &s.scratch_bytes, bytes);
+  StreamState* s = state();
+  if (s == nullptr) return nullptr;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return grow(&s->scratch, bytes);
 }
 
 void* queue_pool(size_t bytes) {
-  DeviceState& s = state();
-  return grow(&s.queue, &s.queue_bytes, bytes);
+  StreamState* s = state();
+  if (s == nullptr) return nullptr;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return grow(&s->queue, bytes);
 }
 
 Error hip_failure(hipError_t err, const char* what) {
@@ -152,33 +223,31 @@ void* awkward_hip_get_stream(void) { return (void*)t_stream; }
 void awkward_hip_set_deferred_errors(int deferred) {
   std::lock_guard<std::mutex> lock(g_mutex);
   g_deferred = deferred != 0;
-  g_invocations.clear();
+  for (auto& kv : g_states) kv.second->invocations.clear();
 }
 
 Error awkward_hip_error_poll(void) {
-  const uint64_t key = take_error();
+  StreamState* s = state();
+  if (s == nullptr) return Error{"no device state for this stream", "awkward_amd/csrc/runtime.hip", kSliceNone,
+                                 kSliceNone};
+  const uint64_t key = read_and_clear(s);
+  std::lock_guard<std::mutex> lock(g_mutex);
   Error out = success();
-  if (key != kNoError) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    const uint32_t inv = (uint32_t)(key >> 48);
-    const int code = error_code(key);
-    if (inv < g_invocations.size() && g_invocations[inv].messages != nullptr) {
-      out = Error{g_invocations[inv].messages[code], g_invocations[inv].kernel, error_identity(key), kSliceNone};
-    } else {
-      out = Error{"kernel error (unknown invocation)", "awkward_amd/csrc/runtime.hip", error_identity(key), kSliceNone};
-    }
+  if (s->pending.str != nullptr) {
+    out = s->pending;
+    s->pending = success();
+  } else if (key != kNoError) {
+    out = describe(s, key);
   }
-  {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    g_invocations.clear();
-  }
+  s->invocations.clear();
   return out;
 }
 
-const char* awkward_hip_version(void) { return "awkward_hip 0.1 (gfx950)"; }
+const char* awkward_hip_version(void) { return "awkward_hip 0.2 (gfx950)"; }
 
-// Size the scratch pool ahead of time (e.g. before capturing a hipGraph, in
-// which no allocation may happen).  Returns 0 on success.
+// Size the current stream's scratch pool ahead of time (e.g. before capturing
+// a hipGraph on that stream, in which no allocation may happen); also creates
+// the stream's state.  Returns 0 on success.
 int awkward_hip_reserve_scratch(int64_t bytes) {
   return scratch((size_t)bytes) != nullptr ? 0 : 1;
 }
@@ -186,6 +255,18 @@ int awkward_hip_reserve_scratch(int64_t bytes) {
 // The same for the work-queue pool (the batch reductions' long-list queue).
 int awkward_hip_reserve_queue(int64_t bytes) {
   return queue_pool((size_t)bytes) != nullptr ? 0 : 1;
+}
+
+// Bytes held by the current stream's pools: current buffers and retired
+// ones (diagnostics; out[0] scratch, out[1] queue, out[2] retired buffers).
+void awkward_hip_pool_bytes(int64_t* out) {
+  StreamState* s = state();
+  out[0] = out[1] = out[2] = 0;
+  if (s == nullptr) return;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  out[0] = (int64_t)s->scratch.bytes;
+  out[1] = (int64_t)s->queue.bytes;
+  out[2] = (int64_t)(s->scratch.retired.size() + s->queue.retired.size());
 }
 
 }  // extern "C"

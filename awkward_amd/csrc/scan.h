@@ -176,9 +176,10 @@ template <int BLOCK, int IPT, class Body>
 Error launch_lookback_scan(const Body& body, int64_t n, uint32_t inv, void* scratch_base) {
   const int64_t ntiles = ceil_div(n, (int64_t)BLOCK * IPT);
   ScanScratch s = scan_scratch_at(scratch_base);
+  AKHIP_ERROR_WORD(ew);
   hipStream_t st = stream();
   scan_prep_kernel<0><<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(s, ntiles);
-  lookback_scan_kernel<BLOCK, IPT, Body><<<(unsigned)ntiles, BLOCK, 0, st>>>(body, s, n, error_word(), inv);
+  lookback_scan_kernel<BLOCK, IPT, Body><<<(unsigned)ntiles, BLOCK, 0, st>>>(body, s, n, ew, inv);
   AKHIP_LAUNCH_CHECK("lookback_scan_kernel");
   return success();
 }

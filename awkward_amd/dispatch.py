@@ -118,6 +118,27 @@ def reduce_sum(out, data, offsets, n, variant):
     return f(device_address(out), device_address(data), device_address(offsets), n, variant)
 
 
+def reduce_prod(out, data, offsets, n, variant):
+    from awkward_amd._kernels import device_address, set_stream
+
+    on = np.dtype(_np(out.dtype)).name
+    inn = np.dtype(_np(data.dtype)).name
+    f = _fn(f"awkward_hip_reduce_prod_{on}_{inn}", [P, P, P, I64, INT])
+    set_stream()
+    return f(device_address(out), device_address(data), device_address(offsets), n, variant)
+
+
+def reduce_anyall(kind, out, mask, data, offsets, n, variant):
+    """kind "any" (awkward_reduce_sum_bool) or "all" (awkward_reduce_prod_bool)."""
+    from awkward_amd._kernels import device_address, set_stream
+
+    tn = np.dtype(_np(data.dtype)).name
+    f = _fn(f"awkward_hip_reduce_{kind}_masked_{tn}", [P, P, P, P, I64, INT])
+    set_stream()
+    return f(device_address(out), 0 if mask is None else device_address(mask), device_address(data),
+             device_address(offsets), n, variant)
+
+
 def reduce_extremum(kind, out, mask, data, offsets, n, identity, variant):
     from awkward_amd._kernels import device_address, set_stream
 

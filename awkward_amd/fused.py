@@ -226,7 +226,12 @@ class Concurrent:
 
 class Graph:
     """Captures fn() (which must not synchronise or allocate through the
-    library) into a hipGraph and replays it."""
+    library) into a hipGraph and replays it.
+
+    The warm-up runs and the capture use one stream, so the library's
+    per-stream state and scratch pools are allocated (at the sizes fn needs)
+    before the capture begins; the captured kernels keep using that stream's
+    pools, which are never freed while the process runs."""
 
     def __init__(self, fn, warmup=2):
         self.fn = fn
@@ -237,8 +242,9 @@ class Graph:
                 fn()
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
+        self.stream = s
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with torch.cuda.graph(self.graph, stream=s):
             self.result = fn()
         torch.cuda.synchronize()
 

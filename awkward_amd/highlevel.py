@@ -1,7 +1,8 @@
 """User-facing API of the jagged-array path: ``Array`` and the ``ak.*``
 functions the configs use, with the reference's signatures and defaults:
 
-* ``sum`` (``operations/ak_sum.py``: ``mask_identity=False``),
+* ``sum`` / ``prod`` / ``any`` / ``all`` (``operations/ak_sum.py``,
+  ``ak_prod.py``, ``ak_any.py``, ``ak_all.py``: ``mask_identity=False``),
   ``max`` / ``min`` (``ak_max.py`` / ``ak_min.py``: ``mask_identity=True``,
   ``initial=None``), ``argmax`` / ``argmin`` (``mask_identity=True``),
   ``count`` (``mask_identity=False``), ``num`` (``ak_num.py:133-142``);
@@ -218,10 +219,50 @@ def _getitem_jagged(layout, item):
     return target._getitem_jagged(item)
 
 
+# ---- axes ------------------------------------------------------------------------
+def _posaxis(layout, axis, name):
+    """The non-negative list depth an axis names (negative axes count from
+    the numbers, as the reference's regularize_axis / maybe_posaxis do for a
+    non-branching layout): 0 is the outer array, depth - 1 the innermost
+    lists."""
+    depth = layout.depth
+    posaxis = axis + depth if axis < 0 else axis
+    if not 0 <= posaxis < depth:
+        raise ValueError(f"axis={axis} exceeds the depth of the nested list structure (which is {depth})")
+    if posaxis == 0:
+        raise NotImplementedError(f"ak.{name} at axis=0 (the outer array) is not on the hip jagged-array path")
+    return posaxis
+
+
+def _as_list_offset(layout):
+    if isinstance(layout, (C.ListArray, C.RegularArray)):
+        layout = layout.to_ListOffsetArray64()
+    if not isinstance(layout, C.ListOffsetArray):
+        raise NotImplementedError(f"{type(layout).__name__} at a list axis on the hip jagged-array path")
+    return layout
+
+
+def _at_axis(layout, posaxis, fn):
+    """fn(list level) at list depth `posaxis` (1: this level), the outer
+    levels kept around the result with their zero-based offsets."""
+    layout = _as_list_offset(layout)
+    if posaxis == 1:
+        return fn(layout)
+    offsets, trimmed = layout._zero_based()
+    return C.ListOffsetArray(offsets, _at_axis(trimmed, posaxis - 1, fn), layout.backend)
+
+
+def _innermost(layout, axis, name):
+    """Operations that act on the numbers' own lists (argsort, sort) take
+    axis=-1 or the positive axis naming the innermost list level."""
+    posaxis = _posaxis(layout, axis, name)
+    if posaxis != layout.depth - 1:
+        raise NotImplementedError(f"ak.{name} at axis={axis} (not the innermost lists) is not on the hip "
+                                  "jagged-array path")
+    return posaxis
+
+
 # ---- reductions -------------------------------------------------------------------
-def _check_axis(axis):
-    if axis not in (-1, 1):
-        raise NotImplementedError("the hip jagged-array path reduces at axis=-1 (lists of numbers)")
 
 
 def _out(result, highlevel):
@@ -276,66 +317,69 @@ def count(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=Tr
     return _out(_reduce(array, R.Count(), keepdims, mask_identity, axis), highlevel)
 
 
+def prod(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.Prod(), keepdims, mask_identity, axis), highlevel)
+
+
+def any(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.Any(), keepdims, mask_identity, axis), highlevel)
+
+
+def all(array, axis=None, keepdims=False, mask_identity=False, *, highlevel=True, behavior=None, attrs=None):
+    return _out(_reduce(array, R.All(), keepdims, mask_identity, axis), highlevel)
+
+
 def num(array, axis=1, *, highlevel=True, behavior=None, attrs=None):
-    """List lengths (ak_num.py:133-142: stops - starts)."""
-    if axis not in (1, -1):
-        raise NotImplementedError("ak.num at axis=1 only")
+    """List lengths at `axis` (ak_num.py:133-142: stops - starts of that
+    level; the levels above keep their structure)."""
     layout = array.layout
-    nplike = layout.nplike
-    if isinstance(layout, C.ListOffsetArray):
-        starts, stops = layout.starts, layout.stops
-    elif isinstance(layout, C.ListArray):
-        starts, stops = layout.starts, layout.stops
-    else:
-        raise NotImplementedError("ak.num of a non-list array")
-    if nplike.dtype(starts) != np.dtype(np.int64):
-        starts, stops = nplike.asarray(starts, np.int64), nplike.asarray(stops, np.int64)
-    return _out(Array(C.NumpyArray(nplike.subtract(stops, starts), layout.backend)), highlevel)
+    posaxis = _posaxis(layout, axis, "num")
+
+    def lengths(level):
+        nplike = level.nplike
+        starts, stops = level.starts, level.stops
+        if nplike.dtype(starts) != np.dtype(np.int64):
+            starts, stops = nplike.asarray(starts, np.int64), nplike.asarray(stops, np.int64)
+        return C.NumpyArray(nplike.subtract(stops, starts), level.backend)
+
+    if posaxis == 1 and isinstance(layout, (C.ListOffsetArray, C.ListArray)):
+        return _out(Array(lengths(layout)), highlevel)
+    return _out(Array(_at_axis(layout, posaxis, lengths)), highlevel)
 
 
 def local_index(array, axis=-1, *, highlevel=True, behavior=None, attrs=None):
-    _check_axis(axis)
     layout = array.layout
-    if isinstance(layout, (C.ListArray, C.RegularArray)):
-        layout = layout.to_ListOffsetArray64()
-    return _out(Array(layout._local_index()), highlevel)
+    posaxis = _posaxis(layout, axis, "local_index")
+    return _out(Array(_at_axis(layout, posaxis, lambda level: level._local_index())), highlevel)
 
 
 def combinations(array, n, replacement=False, axis=1, fields=None, *, highlevel=True, behavior=None, attrs=None):
-    if axis not in (1, -1):
-        raise NotImplementedError("ak.combinations at axis=1 only")
     layout = array.layout
-    if isinstance(layout, (C.ListArray, C.RegularArray)):
-        layout = layout.to_ListOffsetArray64()
+    posaxis = _posaxis(layout, axis, "combinations")
     if fields is not None and len(fields) != n:
         raise ValueError("if provided, the length of 'fields' must be 'n'")
-    return _out(Array(layout._combinations(n, replacement, fields)), highlevel)
+    return _out(Array(_at_axis(layout, posaxis, lambda level: level._combinations(n, replacement, fields))),
+                highlevel)
 
 
 def argcombinations(array, n, replacement=False, axis=1, fields=None, *, highlevel=True, behavior=None,
                     attrs=None):
     """ak.argcombinations (operations/ak_argcombinations.py): the combinations
     of each list's local indices."""
-    if axis not in (1, -1):
-        raise NotImplementedError("ak.argcombinations at axis=1 only")
-    return combinations(local_index(array, axis=-1), n, replacement=replacement, axis=axis, fields=fields,
+    return combinations(local_index(array, axis=axis), n, replacement=replacement, axis=axis, fields=fields,
                         highlevel=highlevel)
 
 
 def argsort(array, axis=-1, ascending=True, stable=True, *, highlevel=True, behavior=None, attrs=None):
-    _check_axis(axis)
     layout = array.layout
-    if isinstance(layout, C.ListArray):
-        layout = layout.to_ListOffsetArray64()
-    return _out(Array(layout._argsort_local(ascending, stable)), highlevel)
+    posaxis = _innermost(layout, axis, "argsort")
+    return _out(Array(_at_axis(layout, posaxis, lambda level: level._argsort_local(ascending, stable))), highlevel)
 
 
 def sort(array, axis=-1, ascending=True, stable=True, *, highlevel=True, behavior=None, attrs=None):
-    _check_axis(axis)
     layout = array.layout
-    if isinstance(layout, C.ListArray):
-        layout = layout.to_ListOffsetArray64()
-    return _out(Array(layout._sort_local(ascending, stable)), highlevel)
+    posaxis = _innermost(layout, axis, "sort")
+    return _out(Array(_at_axis(layout, posaxis, lambda level: level._sort_local(ascending, stable))), highlevel)
 
 
 # ---- construction and backends ------------------------------------------------------
@@ -361,7 +405,7 @@ def _layout_from_python(items, backend, dtype):
     IndexedOptionArray(int64 index, -1 for None) over the present values --
     the layouts the reference's ak.from_iter builds."""
     nplike = backend.nplike
-    if any(v is None for v in items):
+    if builtins.any(v is None for v in items):
         present = [v for v in items if v is not None]
         index = np.full(len(items), -1, dtype=np.int64)
         index[[i for i, v in enumerate(items) if v is not None]] = np.arange(len(present))

@@ -10,6 +10,12 @@
   ``apply_positional_corrections`` (:85-129) subtracts the list starts; bool
   data is viewed as int8.
 * ``Count`` (:254-279).
+* ``Prod`` (:676-758): the types of ``Sum``; booleans reduce with
+  ``awkward_reduce_prod_bool`` and the result is cast to int64.
+* ``Any`` / ``All`` (:760-858): ``awkward_reduce_sum_bool`` /
+  ``awkward_reduce_prod_bool`` of any dtype, a bool result.  ``Max`` /
+  ``Min`` of booleans are these two kernels (:1046-1062, :899-913), and keep
+  the bool dtype.
 
 ``apply`` returns ``(result, mask)``.  On a backend with ``fused = True``
 (the hip backend) the result comes from one tuned kernel per reduction --
@@ -93,7 +99,8 @@ class _Extremum(Reducer):
     def apply(self, array, offsets, starts, shifts, outlength, mask=False):
         dtype = array.dtype
         if dtype == np.dtype(np.bool_):
-            raise NotImplementedError(f"ak.{self.name} of booleans on the hip backend")
+            # _reducers.py:1046-1062 (max: any) and :899-913 (min: all)
+            return _any_all("any" if self.is_max else "all", array, offsets, outlength, mask)
         result = array.nplike.empty(outlength, dtype)
         identity = self.identity_for(dtype)
         if dtype.kind == "f":
@@ -179,3 +186,57 @@ class Count(Reducer):
         result = array.nplike.empty(outlength, np.int64)
         array._check(array.backend["awkward_reduce_count_64", np.int64, np.int64](result, offsets, outlength))
         return type(array)(result, array.backend), None
+
+
+def _any_all(kind, array, offsets, outlength, mask):
+    """awkward_reduce_sum_bool (kind "any") or awkward_reduce_prod_bool
+    ("all") into a bool result; on the fused backend with the empty-list
+    mask of mask_identity=True from the same pass."""
+    result = array.nplike.empty(outlength, np.bool_)
+    if _fused(array):
+        from awkward_amd import dispatch
+
+        outmask = array.nplike.empty(outlength, np.int8) if mask else None
+        array._check(dispatch.reduce_anyall(kind, result, outmask, array.data, offsets, outlength,
+                                            _variant(kind, array, outlength, offsets)))
+        return type(array)(result, array.backend), outmask
+    family = "awkward_reduce_sum_bool" if kind == "any" else "awkward_reduce_prod_bool"
+    array._check(array.backend[family, np.bool_, array.dtype, np.int64](result, array.data, offsets, outlength))
+    return type(array)(result, array.backend), None
+
+
+class Prod(Reducer):
+    name = "prod"
+
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
+        nplike = array.nplike
+        dtype = array.dtype
+        if dtype == np.dtype(np.bool_):
+            # _reducers.py:691-714: all-nonzero per list, cast to the promoted integer type
+            flags, _ = _any_all("all", array, offsets, outlength, False)
+            return type(array)(nplike.asarray(flags.data, _promote_integer_rank(dtype)), array.backend), None
+        out_dtype = _promote_integer_rank(dtype)
+        result = nplike.empty(outlength, out_dtype)
+        if _fused(array):
+            from awkward_amd import dispatch
+
+            array._check(dispatch.reduce_prod(result, array.data, offsets, outlength,
+                                              _variant("prod", array, outlength, offsets)))
+        else:
+            array._check(array.backend["awkward_reduce_prod", out_dtype, dtype, np.int64](result, array.data, offsets,
+                                                                                         outlength))
+        return type(array)(result, array.backend), None
+
+
+class Any(Reducer):
+    name = "any"
+
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
+        return _any_all("any", array, offsets, outlength, mask)
+
+
+class All(Reducer):
+    name = "all"
+
+    def apply(self, array, offsets, starts, shifts, outlength, mask=False):
+        return _any_all("all", array, offsets, outlength, mask)

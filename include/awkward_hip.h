@@ -146,6 +146,88 @@ struct Error awkward_reduce_min_float32_float32_64(float* toptr, const float* fr
 /* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_min.cpp:61; contract kernel-specification.yml:4743 */
 struct Error awkward_reduce_min_float64_float64_64(double* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength, double identity);
 
+/* ---- awkward_reduce_prod ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:34; contract kernel-specification.yml:4796 */
+struct Error awkward_reduce_prod_int64_int8_64(int64_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:35; contract kernel-specification.yml:4802 */
+struct Error awkward_reduce_prod_uint64_uint8_64(uint64_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:36; contract kernel-specification.yml:4808 */
+struct Error awkward_reduce_prod_int64_int16_64(int64_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:37; contract kernel-specification.yml:4814 */
+struct Error awkward_reduce_prod_uint64_uint16_64(uint64_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:38; contract kernel-specification.yml:4820 */
+struct Error awkward_reduce_prod_int64_int32_64(int64_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:39; contract kernel-specification.yml:4826 */
+struct Error awkward_reduce_prod_uint64_uint32_64(uint64_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:40; contract kernel-specification.yml:4832 */
+struct Error awkward_reduce_prod_int64_int64_64(int64_t* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:41; contract kernel-specification.yml:4838 */
+struct Error awkward_reduce_prod_uint64_uint64_64(uint64_t* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:42; contract kernel-specification.yml:4844 */
+struct Error awkward_reduce_prod_float32_float32_64(float* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:43; contract kernel-specification.yml:4850 */
+struct Error awkward_reduce_prod_float64_float64_64(double* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:44; contract kernel-specification.yml:4856 */
+struct Error awkward_reduce_prod_int32_int8_64(int32_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:45; contract kernel-specification.yml:4862 */
+struct Error awkward_reduce_prod_uint32_uint8_64(uint32_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:46; contract kernel-specification.yml:4868 */
+struct Error awkward_reduce_prod_int32_int16_64(int32_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:47; contract kernel-specification.yml:4874 */
+struct Error awkward_reduce_prod_uint32_uint16_64(uint32_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:48; contract kernel-specification.yml:4880 */
+struct Error awkward_reduce_prod_int32_int32_64(int32_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod.cpp:49; contract kernel-specification.yml:4886 */
+struct Error awkward_reduce_prod_uint32_uint32_64(uint32_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength);
+
+/* ---- awkward_reduce_sum_bool ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:42; contract kernel-specification.yml:5479 */
+struct Error awkward_reduce_sum_bool_bool_64(bool* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:43; contract kernel-specification.yml:5485 */
+struct Error awkward_reduce_sum_bool_int8_64(bool* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:44; contract kernel-specification.yml:5491 */
+struct Error awkward_reduce_sum_bool_uint8_64(bool* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:45; contract kernel-specification.yml:5497 */
+struct Error awkward_reduce_sum_bool_int16_64(bool* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:46; contract kernel-specification.yml:5503 */
+struct Error awkward_reduce_sum_bool_uint16_64(bool* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:47; contract kernel-specification.yml:5509 */
+struct Error awkward_reduce_sum_bool_int32_64(bool* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:48; contract kernel-specification.yml:5515 */
+struct Error awkward_reduce_sum_bool_uint32_64(bool* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:49; contract kernel-specification.yml:5521 */
+struct Error awkward_reduce_sum_bool_int64_64(bool* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:50; contract kernel-specification.yml:5527 */
+struct Error awkward_reduce_sum_bool_uint64_64(bool* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:51; contract kernel-specification.yml:5533 */
+struct Error awkward_reduce_sum_bool_float32_64(bool* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_sum_bool.cpp:52; contract kernel-specification.yml:5539 */
+struct Error awkward_reduce_sum_bool_float64_64(bool* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength);
+
+/* ---- awkward_reduce_prod_bool ---- */
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:34; contract kernel-specification.yml:4933 */
+struct Error awkward_reduce_prod_bool_bool_64(bool* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:35; contract kernel-specification.yml:4939 */
+struct Error awkward_reduce_prod_bool_int8_64(bool* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:36; contract kernel-specification.yml:4945 */
+struct Error awkward_reduce_prod_bool_uint8_64(bool* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:37; contract kernel-specification.yml:4951 */
+struct Error awkward_reduce_prod_bool_int16_64(bool* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:38; contract kernel-specification.yml:4957 */
+struct Error awkward_reduce_prod_bool_uint16_64(bool* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:39; contract kernel-specification.yml:4963 */
+struct Error awkward_reduce_prod_bool_int32_64(bool* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:40; contract kernel-specification.yml:4969 */
+struct Error awkward_reduce_prod_bool_uint32_64(bool* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:41; contract kernel-specification.yml:4975 */
+struct Error awkward_reduce_prod_bool_int64_64(bool* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:42; contract kernel-specification.yml:4981 */
+struct Error awkward_reduce_prod_bool_uint64_64(bool* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:43; contract kernel-specification.yml:4987 */
+struct Error awkward_reduce_prod_bool_float32_64(bool* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength);
+/* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_prod_bool.cpp:44; contract kernel-specification.yml:4993 */
+struct Error awkward_reduce_prod_bool_float64_64(bool* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength);
+
 /* ---- awkward_reduce_argmax ---- */
 /* replaces awkward-cpp/src/cpu-kernels/awkward_reduce_argmax.cpp:43; contract kernel-specification.yml:4227 */
 struct Error awkward_reduce_argmax_int8_64(int64_t* toptr, const int8_t* fromptr, const int64_t* offsets, const int64_t* starts, int64_t outlength);

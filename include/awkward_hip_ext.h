@@ -20,11 +20,22 @@ extern "C" {
 #endif
 
 /* ---- runtime ------------------------------------------------------------ */
-/* Grow the per-device scratch pool to at least `bytes` now (before capturing
-   a hipGraph whose kernels use scratch).  0 on success. */
+/* State is kept per (device, stream): each stream that calls the library has
+   its own device error word, deferred-mode invocation table and scratch
+   pools, so work on different streams never shares scratch or errors.  A
+   pool that grows retires its old buffer instead of freeing it (in-flight
+   kernels and captured hipGraphs may still use it); retired buffers stay
+   allocated until the process ends.  No entry point synchronises the device
+   or frees memory to grow a pool. */
+/* Grow the current stream's scratch pool to at least `bytes` now (and create
+   the stream's state), e.g. before capturing a hipGraph on that stream.
+   0 on success. */
 int awkward_hip_reserve_scratch(int64_t bytes);
 /* The same for the work-queue pool (the batch reductions' long-list queue). */
 int awkward_hip_reserve_queue(int64_t bytes);
+/* Diagnostics: out[0] = bytes of the current stream's scratch pool, out[1] =
+   of its queue pool, out[2] = number of retired pool buffers. */
+void awkward_hip_pool_bytes(int64_t* out);
 /* Number of reduce tile variants (threads x lists per thread x LDS stage). */
 int awkward_hip_reduce_variant_count(void);
 
@@ -64,6 +75,48 @@ struct Error awkward_hip_reduce_sum_float64_uint64(double* toptr, const uint64_t
 struct Error awkward_hip_reduce_sum_float64_float32(double* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_sum_int64_bool(int64_t* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_sum_int32_bool(int32_t* toptr, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+/* prod:  awkward_reduce_prod_<OUT>_<IN>_64 with the variant's tile. */
+struct Error awkward_hip_reduce_prod_int64_int8(int64_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint64_uint8(uint64_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int64_int16(int64_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint64_uint16(uint64_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int64_int32(int64_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint64_uint32(uint64_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int64_int64(int64_t* toptr, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint64_uint64(uint64_t* toptr, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_float32_float32(float* toptr, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_float64_float64(double* toptr, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int32_int8(int32_t* toptr, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint32_uint8(uint32_t* toptr, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int32_int16(int32_t* toptr, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint32_uint16(uint32_t* toptr, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_int32_int32(int32_t* toptr, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_prod_uint32_uint32(uint32_t* toptr, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+/* any / all (ak.any, ak.all; ak.max / ak.min of booleans): awkward_reduce_sum_bool_<T>_64 /
+   awkward_reduce_prod_bool_<T>_64 with the variant's tile, plus the empty-list mask of
+   mask_identity=True (mask[k] = 1 for an empty list; NULL: not written). */
+struct Error awkward_hip_reduce_any_masked_bool(bool* toptr, int8_t* mask, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_int8(bool* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_uint8(bool* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_int16(bool* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_uint16(bool* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_int32(bool* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_uint32(bool* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_int64(bool* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_uint64(bool* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_float32(bool* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_any_masked_float64(bool* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_bool(bool* toptr, int8_t* mask, const bool* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_int8(bool* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_uint8(bool* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_int16(bool* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_uint16(bool* toptr, int8_t* mask, const uint16_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_int32(bool* toptr, int8_t* mask, const int32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_uint32(bool* toptr, int8_t* mask, const uint32_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_int64(bool* toptr, int8_t* mask, const int64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_uint64(bool* toptr, int8_t* mask, const uint64_t* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_float32(bool* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, int64_t outlength, int variant);
+struct Error awkward_hip_reduce_all_masked_float64(bool* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_max_masked_int8(int8_t* toptr, int8_t* mask, const int8_t* fromptr, const int64_t* offsets, int64_t outlength, int8_t identity, int variant);
 struct Error awkward_hip_reduce_max_masked_uint8(uint8_t* toptr, int8_t* mask, const uint8_t* fromptr, const int64_t* offsets, int64_t outlength, uint8_t identity, int variant);
 struct Error awkward_hip_reduce_max_masked_int16(int16_t* toptr, int8_t* mask, const int16_t* fromptr, const int64_t* offsets, int64_t outlength, int16_t identity, int variant);

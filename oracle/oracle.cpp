@@ -79,6 +79,48 @@ Error reduce_sum_bool(OUT* toptr, const bool* fromptr, const int64_t* offsets, i
   return ok();
 }
 
+// Integer products wrap modulo 2**bits too (multiplied in the unsigned type).
+template <typename OUT, typename IN>
+inline OUT mul_by(OUT acc, IN x) {
+  if constexpr (std::is_floating_point_v<OUT>) {
+    return acc * static_cast<OUT>(x);
+  } else {
+    using U = std::make_unsigned_t<OUT>;
+    return static_cast<OUT>(static_cast<U>(acc) * static_cast<U>(static_cast<OUT>(x)));
+  }
+}
+
+// awkward_reduce_prod.cpp:11-24; kernel-specification.yml:4794-4900: acc = 1,
+// acc *= (OUT)x left to right
+template <typename OUT, typename IN>
+Error reduce_prod(OUT* toptr, const IN* fromptr, const int64_t* offsets, int64_t outlength) {
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    OUT acc = OUT(1);
+    for (int64_t i = offsets[bin]; i < offsets[bin + 1]; i++) acc = mul_by<OUT, IN>(acc, fromptr[i]);
+    toptr[bin] = acc;
+  }
+  return ok();
+}
+
+// awkward_reduce_sum_bool.cpp / awkward_reduce_prod_bool.cpp;
+// kernel-specification.yml:5477-5550 and :4931-5009: whether any (ANY) or
+// every (!ANY) element of the bin is nonzero (x != 0: NaN counts as nonzero,
+// -0.0 as zero); an empty bin gives false (any) / true (all)
+template <typename IN, bool ANY>
+Error reduce_anyall(bool* toptr, const IN* fromptr, const int64_t* offsets, int64_t outlength) {
+  for (int64_t bin = 0; bin < outlength; bin++) {
+    bool r = !ANY;
+    for (int64_t i = offsets[bin]; i < offsets[bin + 1]; i++) {
+      if ((fromptr[i] != 0) == ANY) {
+        r = ANY;
+        break;
+      }
+    }
+    toptr[bin] = r;
+  }
+  return ok();
+}
+
 // awkward_reduce_max.cpp:8-26: best = identity; best = v > best ? v : best
 template <typename T>
 Error reduce_max(T* toptr, const T* fromptr, const int64_t* offsets, int64_t outlength, T identity) {
@@ -171,6 +213,46 @@ Error awkward_reduce_sum_int64_bool_64(int64_t* t, const bool* f, const int64_t*
 Error awkward_reduce_sum_int32_bool_64(int32_t* t, const bool* f, const int64_t* o, int64_t n) {
   return reduce_sum_bool<int32_t>(t, f, o, n);
 }
+
+#define PROD(NAME, OUT, IN) \
+  Error NAME(OUT* t, const IN* f, const int64_t* o, int64_t n) { return reduce_prod<OUT, IN>(t, f, o, n); }
+PROD(awkward_reduce_prod_int64_int8_64, int64_t, int8_t)
+PROD(awkward_reduce_prod_uint64_uint8_64, uint64_t, uint8_t)
+PROD(awkward_reduce_prod_int64_int16_64, int64_t, int16_t)
+PROD(awkward_reduce_prod_uint64_uint16_64, uint64_t, uint16_t)
+PROD(awkward_reduce_prod_int64_int32_64, int64_t, int32_t)
+PROD(awkward_reduce_prod_uint64_uint32_64, uint64_t, uint32_t)
+PROD(awkward_reduce_prod_int64_int64_64, int64_t, int64_t)
+PROD(awkward_reduce_prod_uint64_uint64_64, uint64_t, uint64_t)
+PROD(awkward_reduce_prod_float32_float32_64, float, float)
+PROD(awkward_reduce_prod_float64_float64_64, double, double)
+PROD(awkward_reduce_prod_int32_int8_64, int32_t, int8_t)
+PROD(awkward_reduce_prod_uint32_uint8_64, uint32_t, uint8_t)
+PROD(awkward_reduce_prod_int32_int16_64, int32_t, int16_t)
+PROD(awkward_reduce_prod_uint32_uint16_64, uint32_t, uint16_t)
+PROD(awkward_reduce_prod_int32_int32_64, int32_t, int32_t)
+PROD(awkward_reduce_prod_uint32_uint32_64, uint32_t, uint32_t)
+#undef PROD
+
+#define ANYALL(TNAME, T)                                                                             \
+  Error awkward_reduce_sum_bool_##TNAME##_64(bool* t, const T* f, const int64_t* o, int64_t n) {     \
+    return reduce_anyall<T, true>(t, f, o, n);                                                       \
+  }                                                                                                  \
+  Error awkward_reduce_prod_bool_##TNAME##_64(bool* t, const T* f, const int64_t* o, int64_t n) {    \
+    return reduce_anyall<T, false>(t, f, o, n);                                                      \
+  }
+ANYALL(bool, bool)
+ANYALL(int8, int8_t)
+ANYALL(uint8, uint8_t)
+ANYALL(int16, int16_t)
+ANYALL(uint16, uint16_t)
+ANYALL(int32, int32_t)
+ANYALL(uint32, uint32_t)
+ANYALL(int64, int64_t)
+ANYALL(uint64, uint64_t)
+ANYALL(float32, float)
+ANYALL(float64, double)
+#undef ANYALL
 
 #define EXTREMA(TNAME, T)                                                                     \
   Error awkward_reduce_max_##TNAME##_##TNAME##_64(T* t, const T* f, const int64_t* o, int64_t n, \

@@ -69,6 +69,62 @@ def test_reduce_sum(out, inp):
         assert (np.abs(h[0].astype(np.float64) - d[0].astype(np.float64)) <= bound).all()
 
 
+@pytest.mark.parametrize("out,inp", [
+    ("int64", "int8"), ("uint64", "uint8"), ("int64", "int16"), ("uint64", "uint16"), ("int64", "int32"),
+    ("uint64", "uint32"), ("int64", "int64"), ("uint64", "uint64"), ("float32", "float32"), ("float64", "float64"),
+    ("int32", "int8"), ("uint32", "uint8"), ("int32", "int16"), ("uint32", "uint16"), ("int32", "int32"),
+    ("uint32", "uint32"),
+])
+def test_reduce_prod(out, inp):
+    """Integer products wrap bit-exactly; float products (values of magnitude
+    0.9-1.1, so long lists neither overflow nor underflow) are bit-exact on
+    serial tiles and otherwise within 2 * n * eps * |product| (each order's
+    relative rounding bound is n * eps)."""
+    rng = np.random.default_rng(13)
+    for label, counts in layouts(rng):
+        off = offsets_from_counts(counts)
+        N = int(off[-1])
+        if np.dtype(inp).kind == "f":
+            x = (rng.uniform(0.9, 1.1, N) * rng.choice([-1.0, 1.0], N)).astype(inp)
+        else:
+            x = values(rng, inp, N)
+        name = f"awkward_reduce_prod_{out}_{inp}_64"
+        h, d, he, de = run_both(name, [np.zeros(len(counts), out), x, off, len(counts)])
+        assert he.str is None and de.str is None
+        if np.dtype(out).kind != "f" or label == "short":
+            assert same_bits(h[0], d[0]), (name, label)
+            continue
+        eps = np.finfo(np.dtype(out)).eps
+        bound = 2 * counts * eps * np.abs(h[0].astype(np.float64))
+        assert (np.abs(h[0].astype(np.float64) - d[0].astype(np.float64)) <= bound).all()
+
+
+@pytest.mark.parametrize("family", ["sum_bool", "prod_bool"])
+@pytest.mark.parametrize("t", ["bool"] + ALL_TYPES)
+def test_reduce_any_all(family, t):
+    """awkward_reduce_sum_bool (any nonzero) / prod_bool (all nonzero):
+    bit-exact, with zeros, -0.0 and NaN in the data and all-zero /
+    all-nonzero lists of every length class."""
+    rng = np.random.default_rng(14)
+    for _, counts in layouts(rng):
+        off = offsets_from_counts(counts)
+        N = int(off[-1])
+        if t == "bool":
+            x = rng.random(N) < 0.5
+        else:
+            x = values(rng, t, N)
+            x[rng.random(N) < 0.5] = 0
+        # whole lists of zeros or of nonzeros
+        for i in range(0, len(counts), 7):
+            x[off[i]:off[i + 1]] = 0
+        for i in range(3, len(counts), 11):
+            x[off[i]:off[i + 1]] = 1
+        name = f"awkward_reduce_{family}_{t}_64"
+        h, d, he, de = run_both(name, [np.zeros(len(counts), np.bool_), x, off, len(counts)])
+        assert he.str is None and de.str is None
+        assert same_bits(h[0], d[0]), name
+
+
 @pytest.mark.parametrize("out", ["int64", "int32"])
 def test_reduce_sum_bool(out):
     rng = np.random.default_rng(12)
