@@ -17,6 +17,9 @@
 // per-element work is coalesced over elements with the owning list found by
 // a binary search in the workgroup's LDS copy of the offsets.
 
+#include <cstdio>
+#include <mutex>
+
 #include "errors.h"
 #include "scan.h"
 
@@ -455,9 +458,91 @@ Error indexed_nextcarry(int64_t* tocarry, const C* fromindex, int64_t lenindex, 
 
 }  // namespace akhip
 
+// ---- next_range without allocation ---------------------------------------------------
+// The look-back scan's tile counter and status words live in this file-scope
+// device array instead of the stream's pool, for callers that cannot
+// allocate (hipGraph capture, serving loops).  Each device holds its own copy
+// (1 MiB) until the process ends.  Host code never passes the array's name to
+// a kernel: its device address comes from hipGetSymbolAddress, once per
+// device.  Layout: [0, 256) the scan's error word, [256, 512) the tile
+// counter, then one 64-bit status word per tile.
+namespace {
+constexpr int64_t kNoAllocBytes = int64_t(1) << 20;
+constexpr int64_t kNoAllocMaxTiles = (kNoAllocBytes - 512) / 8;
+constexpr int64_t kNoAllocMaxLists = kNoAllocMaxTiles * 256 * 8;  // tiles of kScanBlock x kScanIpt lists
+}  // namespace
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocBytes / 8];
+
+namespace akhip {
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mutex;
+char* g_noalloc_ptr[kNoAllocDevices];
+
+// The current device's address of `scratch` (nullptr on failure).
+char* noalloc_scratch(hipError_t* err) {
+  int d = 0;
+  *err = hipGetDevice(&d);
+  if (*err != hipSuccess || d < 0 || d >= kNoAllocDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (g_noalloc_ptr[d] == nullptr) {
+    void* p = nullptr;
+    *err = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (*err != hipSuccess) return nullptr;
+    g_noalloc_ptr[d] = (char*)p;
+  }
+  return g_noalloc_ptr[d];
+}
+
+__global__ void noalloc_prep_kernel(uint64_t* err, ScanScratch s, int64_t ntiles) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *err = kNoError;
+    *s.counter = 0u;
+  }
+  if (i < ntiles) s.status[i] = 0;
+}
+}  // namespace
+}  // namespace akhip
+
 using namespace akhip;
 
 extern "C" {
+
+// awkward_ListArray64_getitem_next_range_64 with the scan scratch in the
+// file-scope array above: no hipMalloc, hipFree or host synchronisation.
+// At most kNoAllocMaxLists (268,304,384) lists; calls that overlap on one
+// device share the array and race.
+Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, int64_t* tocarry,
+                                                            const int64_t* fromstarts, const int64_t* fromstops,
+                                                            int64_t lenstarts, int64_t start, int64_t stop,
+                                                            int64_t step) {
+  static thread_local char msg[160];
+  if (lenstarts > kNoAllocMaxLists) {
+    snprintf(msg, sizeof(msg), "lenstarts %lld exceeds the no-allocation scratch (at most %lld lists)",
+             (long long)lenstarts, (long long)kNoAllocMaxLists);
+    return failure(msg, kSliceNone, kSliceNone, kFile);
+  }
+  hipStream_t st = stream();
+  if (lenstarts <= 0) {
+    fill_c<int64_t><<<1, 1, 0, st>>>(tooffsets, (int64_t)0);
+    AKHIP_LAUNCH_CHECK("fill_c");
+    return success();
+  }
+  hipError_t herr = hipSuccess;
+  char* base = noalloc_scratch(&herr);
+  if (base == nullptr) return hip_failure(herr, "hipGetSymbolAddress of the no-allocation scratch");
+  uint64_t* err = (uint64_t*)base;
+  const ScanScratch s = scan_scratch_at(base + 256);
+  const int64_t ntiles = ceil_div(lenstarts, (int64_t)kScanBlock * kScanIpt);
+  noalloc_prep_kernel<<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(err, s, ntiles);
+  RangeBody<int64_t> body{tooffsets, tocarry, fromstarts, fromstops, start, stop, step};
+  lookback_scan_kernel<kScanBlock, kScanIpt, RangeBody<int64_t>><<<(unsigned)ntiles, kScanBlock, 0, st>>>(
+      body, s, lenstarts, err, 0);
+  AKHIP_LAUNCH_CHECK("lookback_scan_kernel (no allocation)");
+  return success();
+}
 
 Error awkward_localindex_64(int64_t* toindex, int64_t length) {
   if (length <= 0) return success();

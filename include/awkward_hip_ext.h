@@ -184,6 +184,18 @@ struct Error awkward_hip_reduce_argmin_shifts_uint64(int64_t* toptr, int8_t* mas
 struct Error awkward_hip_reduce_argmin_shifts_float32(int64_t* toptr, int8_t* mask, const float* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
 struct Error awkward_hip_reduce_argmin_shifts_float64(int64_t* toptr, int8_t* mask, const double* fromptr, const int64_t* offsets, const int64_t* starts, const int64_t* shifts, int64_t outlength, int variant);
 
+/* ---- without allocation ------------------------------------------------------
+   No counterpart in the reference: for callers that cannot allocate device
+   memory (code under hipGraph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  awkward_ListArray64_getitem_next_range_64 with the
+   look-back scan's scratch (tile counter and status words) in a file-scope
+   1 MiB __device__ array instead of the stream's pool; no hipMalloc,
+   hipFree or host synchronisation.  At most 268,304,384 lists (a clear
+   error beyond).  Every call on one device shares that device's array, and
+   each call clears what it uses first: calls that overlap on one device
+   (concurrent streams, threads) race. */
+struct Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, int64_t* tocarry, const int64_t* fromstarts, const int64_t* fromstops, int64_t lenstarts, int64_t start, int64_t stop, int64_t step);
+
 /* ---- gathers and small host-layer steps ------------------------------------
    take_<bytes>: out[i] = data[index[i]] (NumpyArray._carry, numpyarray.py:367-373).
    bool_to_index: the jagged boolean mask -> jagged int index of

@@ -14,10 +14,13 @@ _GPU_ORDER = ["test_hip_golden", "test_hip_parity", "test_pipeline", "test_fused
 
 
 def pytest_collection_modifyitems(session, config, items):
-    """GPU files run core kernel parity first, fused/graph forms last."""
+    """GPU files run core kernel parity first, fused/graph forms later, and
+    test_zz_* files after everything else."""
 
     def rank(item):
         mod = item.module.__name__.rsplit(".", 1)[-1]
+        if mod.startswith("test_zz_"):
+            return len(_GPU_ORDER)
         return _GPU_ORDER.index(mod) if mod in _GPU_ORDER else -1
 
     items.sort(key=rank)
