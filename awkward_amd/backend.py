@@ -95,7 +95,7 @@ class HipBackend(KernelBackend):
     def __init__(self, device=None, fused=True):
         from awkward_amd.nplike import TorchHip
 
-        super().__init__(TorchHip(device))
+        super().__init__(TorchHip(device, fuse=fused))
         self.fused = fused
 
     def _make_kernel(self, name: str):

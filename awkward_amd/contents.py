@@ -65,9 +65,23 @@ class Content:
 
 # ---- leaves ---------------------------------------------------------------------
 class NumpyArray(Content):
+    """1-D numbers.  On a fusing nplike the buffer may be a lazy
+    elementwise expression (``raw``); ``data`` writes it out on first use."""
+
     def __init__(self, data, backend):
-        self.data = data
+        self._data = data
         self.backend = backend
+
+    @property
+    def data(self):
+        d = self._data
+        if hasattr(d, "materialize"):
+            d = self._data = d.materialize()
+        return d
+
+    @property
+    def raw(self):
+        return self._data
 
     @property
     def depth(self):
@@ -75,11 +89,11 @@ class NumpyArray(Content):
 
     @property
     def length(self):
-        return self.nplike.length(self.data)
+        return self.nplike.length(self._data)
 
     @property
     def dtype(self):
-        return self.nplike.dtype(self.data)
+        return self.nplike.dtype(self._data)
 
     def __repr__(self):
         return f"NumpyArray(len={self.length}, dtype={self.dtype})"

@@ -580,10 +580,10 @@ Error combinations_length(int64_t* totallen, int64_t* tooffsets, int64_t n, bool
     return success();
   }
   LengthBody<C> body{totallen, tooffsets, starts, stops, n, replacement, length};
-  const int64_t ntiles = ceil_div(length, (int64_t)256 * 8);
+  const int64_t ntiles = scan_tiles(length);
   void* s = scratch(scan_scratch_bytes(ntiles));
   if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
-  return launch_lookback_scan<256, 8>(body, length, 0, s);
+  return launch_lookback_scan(body, length, 0, s);
 }
 
 template <typename C>
@@ -592,7 +592,7 @@ Error combinations(int64_t** tocarry, int64_t* toindex, int64_t*, int64_t n, boo
   if (n <= 0) return success();
   if (n > kMaxN) return failure("combinations with n > 8 are not supported on the hip backend", kSliceNone, kSliceNone, kFile);
   // scratch: per-list first slot (length + 1) followed by the scan state
-  const int64_t ntiles = ceil_div(length > 0 ? length : 1, (int64_t)256 * 8);
+  const int64_t ntiles = scan_tiles(length);
   const size_t off_bytes = ((size_t)(length + 1) * sizeof(int64_t) + 255) / 256 * 256;
   char* s = (char*)scratch(off_bytes + scan_scratch_bytes(ntiles));
   if (!s) return failure("out of device memory for combinations scratch", kSliceNone, kSliceNone, kFile);
@@ -601,7 +601,7 @@ Error combinations(int64_t** tocarry, int64_t* toindex, int64_t*, int64_t n, boo
     zero_two<<<1, 1, 0, stream()>>>(offsets, offsets);
   } else {
     OffsetsBody<C> body{offsets, starts, stops, n, replacement};
-    Error e = launch_lookback_scan<256, 8>(body, length, 0, s + off_bytes);
+    Error e = launch_lookback_scan(body, length, 0, s + off_bytes);
     if (e.str) return e;
   }
   return launch_decode<C>(tocarry, toindex, offsets, starts, stops, length > 0 ? length : 0, n, replacement, 0);

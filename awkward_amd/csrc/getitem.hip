@@ -27,8 +27,6 @@ namespace akhip {
 
 namespace {
 const char* kFile = "awkward_amd/csrc/getitem.hip";
-constexpr int kScanBlock = 256;
-constexpr int kScanIpt = 8;
 
 // ---- helpers ----------------------------------------------------------------
 __global__ void fill_i64(int64_t* p, int64_t v) { *p = v; }
@@ -332,10 +330,9 @@ __global__ void nextcarry_kernel(int64_t* __restrict__ tocarry, const C* __restr
 
 template <class Body>
 Error run_scan(const Body& body, int64_t n, uint32_t inv) {
-  const int64_t ntiles = ceil_div(n, (int64_t)kScanBlock * kScanIpt);
-  void* s = scratch(scan_scratch_bytes(ntiles));
+  void* s = scratch(scan_scratch_bytes(scan_tiles(n)));
   if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
-  return launch_lookback_scan<kScanBlock, kScanIpt>(body, n, inv, s);
+  return launch_lookback_scan(body, n, inv, s);
 }
 
 }  // namespace
@@ -469,7 +466,7 @@ Error indexed_nextcarry(int64_t* tocarry, const C* fromindex, int64_t lenindex, 
 namespace {
 constexpr int64_t kNoAllocBytes = int64_t(1) << 20;
 constexpr int64_t kNoAllocMaxTiles = (kNoAllocBytes - 512) / 8;
-constexpr int64_t kNoAllocMaxLists = kNoAllocMaxTiles * 256 * 8;  // tiles of kScanBlock x kScanIpt lists
+constexpr int64_t kNoAllocMaxLists = kNoAllocMaxTiles * akhip::kScanTile;
 }  // namespace
 
 static __device__ __align__(256) uint64_t scratch[kNoAllocBytes / 8];
@@ -512,7 +509,7 @@ extern "C" {
 
 // awkward_ListArray64_getitem_next_range_64 with the scan scratch in the
 // file-scope array above: no hipMalloc, hipFree or host synchronisation.
-// At most kNoAllocMaxLists (268,304,384) lists; calls that overlap on one
+// At most kNoAllocMaxLists (1,073,217,536) lists; calls that overlap on one
 // device share the array and race.
 Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, int64_t* tocarry,
                                                             const int64_t* fromstarts, const int64_t* fromstops,
@@ -535,10 +532,10 @@ Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, 
   if (base == nullptr) return hip_failure(herr, "hipGetSymbolAddress of the no-allocation scratch");
   uint64_t* err = (uint64_t*)base;
   const ScanScratch s = scan_scratch_at(base + 256);
-  const int64_t ntiles = ceil_div(lenstarts, (int64_t)kScanBlock * kScanIpt);
+  const int64_t ntiles = scan_tiles(lenstarts);
   noalloc_prep_kernel<<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(err, s, ntiles);
   RangeBody<int64_t> body{tooffsets, tocarry, fromstarts, fromstops, start, stop, step};
-  lookback_scan_kernel<kScanBlock, kScanIpt, RangeBody<int64_t>><<<(unsigned)ntiles, kScanBlock, 0, st>>>(
+  lookback_scan_kernel<akhip::kScanBlock, akhip::kScanIpt, RangeBody<int64_t>><<<(unsigned)ntiles, akhip::kScanBlock, 0, st>>>(
       body, s, lenstarts, err, 0);
   AKHIP_LAUNCH_CHECK("lookback_scan_kernel (no allocation)");
   return success();

@@ -337,10 +337,10 @@ Error colmajor(ColMajor& cm, const int64_t* offsets, const int64_t* outer_offset
   }
   set_i64<<<1, 1, 0, st>>>(cm.prefix, 0);
   if (cm.groups > 0) {
-    const int64_t ntiles = ceil_div(cm.groups, (int64_t)256 * 8);
+    const int64_t ntiles = scan_tiles(cm.groups);
     void* s = scratch(scan_scratch_bytes(ntiles));
     if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
-    Error e = launch_lookback_scan<256, 8>(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
+    Error e = launch_lookback_scan(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
     if (e.str) return e;
   }
   AKHIP_LAUNCH_CHECK("colmajor");
@@ -689,10 +689,10 @@ Error colmajor_fast(ColMajor& cm, ColRanks& cr, const int64_t* offsets, const in
   if (cm.groups > 0) {
     cnt_from_bins_kernel<<<grid_for(cm.groups, kBlock, 8), kBlock, 0, st>>>(cm.cnt, cr.binS, cm.groups, maxcount);
     if (with_prefix) {
-      const int64_t ntiles = ceil_div(cm.groups, (int64_t)256 * 8);
+      const int64_t ntiles = scan_tiles(cm.groups);
       void* s = scratch(scan_scratch_bytes(ntiles));
       if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
-      e = launch_lookback_scan<256, 8>(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
+      e = launch_lookback_scan(PrefixBody{cm.cnt, cm.prefix, cm.maxgroup}, cm.groups, 0, s);
       if (e.str) return e;
     }
   }

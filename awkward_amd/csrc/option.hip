@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void adjust_shifts_kernel(int64_t* __restri
 // ---- drivers ---------------------------------------------------------------------
 template <class Body>
 Error scan_elements(const Body& body, int64_t n, uint32_t inv, void* scan_scratch) {
-  return launch_lookback_scan<kBlock, kIpt>(body, n, inv, scan_scratch);
+  return launch_lookback_scan(body, n, inv, scan_scratch);
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -184,7 +184,7 @@ inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 template <class Body>
 Error compact(const Body& body, int64_t n, uint32_t inv = 0) {
   if (n <= 0) return success();
-  void* s = scratch(scan_scratch_bytes(ceil_div(n, (int64_t)kBlock * kIpt)));
+  void* s = scratch(scan_scratch_bytes(scan_tiles(n)));
   if (!s) return failure("out of device memory for scan scratch", kSliceNone, kSliceNone, kFile);
   return scan_elements(body, n, inv, s);
 }
@@ -214,7 +214,7 @@ Error reduce_next(int64_t* nextcarry, int64_t* nextoffsets, int64_t* outindex, c
   const int64_t items = n > 0 ? n : 0;
   // scratch: [total | positions (only for an index carry) | scan state]
   const size_t pos_bytes = index ? round256((size_t)items * sizeof(int64_t)) : 0;
-  const size_t scan_bytes = scan_scratch_bytes(ceil_div(items > 0 ? items : 1, (int64_t)kBlock * kIpt));
+  const size_t scan_bytes = scan_scratch_bytes(scan_tiles(items));
   char* s = (char*)scratch(256 + pos_bytes + scan_bytes);
   if (!s) return failure("out of device memory for option-type scratch", kSliceNone, kSliceNone, kFile);
   int64_t* total = (int64_t*)s;

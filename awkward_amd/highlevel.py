@@ -133,6 +133,24 @@ class Array:
     def __rmul__(self, o):
         return self._binary(np.multiply, o, True)
 
+    def __truediv__(self, o):
+        return self._binary(np.true_divide, o)
+
+    def __rtruediv__(self, o):
+        return self._binary(np.true_divide, o, True)
+
+    def __rand__(self, o):
+        return self._binary(np.bitwise_and, o, True)
+
+    def __ror__(self, o):
+        return self._binary(np.bitwise_or, o, True)
+
+    def __xor__(self, o):
+        return self._binary(np.bitwise_xor, o)
+
+    def __invert__(self):
+        return _broadcast_ufunc(np.invert, (self,))
+
     __hash__ = None
 
 
@@ -143,21 +161,24 @@ def _record_of(layout):
     return node if isinstance(node, C.RecordArray) else None
 
 
-def _leaf(layout):
-    """(list structure, flat data) of a list-of-numbers or flat layout."""
+def _operand(layout):
+    """(list structure, elementwise operand) of a list-of-numbers or flat
+    layout.  A lazy carry of numbers (IndexedArray) is handed over as a
+    gather, which the hip nplike reads in place."""
     if isinstance(layout, C.ListArray):
         layout = layout.to_ListOffsetArray64()
+    offsets = None
     if isinstance(layout, C.ListOffsetArray):
-        inner = layout.content
-        if isinstance(inner, C.IndexedArray):
-            inner = inner.project()
-        if not isinstance(inner, C.NumpyArray):
-            raise NotImplementedError("elementwise operations need lists of numbers")
-        return layout.offsets, inner
+        offsets, layout = layout.offsets, layout.content
+    nplike = layout.nplike
+    if isinstance(layout, C.IndexedArray) and isinstance(layout.content, C.NumpyArray):
+        return offsets, nplike.gathered(layout.content.raw, layout.index)
     if isinstance(layout, C.IndexedArray):
         layout = layout.project()
     if isinstance(layout, C.NumpyArray):
-        return None, layout
+        return offsets, layout.raw
+    if offsets is not None:
+        raise NotImplementedError("elementwise operations need lists of numbers")
     raise NotImplementedError(f"elementwise operation on {layout!r}")
 
 
@@ -170,7 +191,7 @@ def _broadcast_ufunc(ufunc, inputs):
     for x in inputs:
         if isinstance(x, Array):
             backend = x.backend
-            offsets, leaf = _leaf(x.layout)
+            offsets, operand = _operand(x.layout)
             if offsets is not None:
                 if structure is None:
                     structure = offsets
@@ -179,7 +200,7 @@ def _broadcast_ufunc(ufunc, inputs):
                     a, b = nplike.to_numpy(structure), nplike.to_numpy(offsets)
                     if a.shape != b.shape or not (a == b).all():
                         raise ValueError("cannot broadcast nested list")
-            args.append(leaf.data)
+            args.append(operand)
         else:
             args.append(x)
     data = backend.nplike.apply_ufunc(ufunc, *args)
@@ -207,6 +228,9 @@ def _getitem_jagged(layout, item):
     if nplike.dtype(offsets) != np.dtype(np.int64):
         offsets = nplike.asarray(offsets, np.int64)
     if content.dtype == np.dtype(np.bool_):
+        fused = _mask_getitem(layout, offsets, content)
+        if fused is not None:
+            return fused
         index, nextoffsets = nplike.bool_to_jagged_index(offsets, content.data)
         item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
     else:
@@ -217,6 +241,31 @@ def _getitem_jagged(layout, item):
         item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
     target = layout.to_ListArray() if isinstance(layout, C.ListOffsetArray) else layout
     return target._getitem_jagged(item)
+
+
+def _mask_getitem(layout, moffsets, mask):
+    """layout[mask] in one pass on a nplike with ``mask_getitem`` (the hip
+    nplike when fusing): the mask -- a recorded predicate or a boolean
+    column -- is evaluated inside the scan that writes the result's offsets
+    and carry, with jagged_apply's errors.  None where the reference's
+    sequence runs instead (other nplikes, non-int64 list indices)."""
+    nplike = layout.nplike
+    if not getattr(nplike, "fuse", False) or not hasattr(nplike, "mask_getitem"):
+        return None
+    target = layout.to_ListArray() if isinstance(layout, C.ListOffsetArray) else layout
+    if not isinstance(target, C.ListArray):
+        return None
+    starts, stops = target._contig()
+    if nplike.dtype(starts) != np.dtype(np.int64) or nplike.dtype(stops) != np.dtype(np.int64):
+        return None
+    n = nplike.length(moffsets) - 1
+    if n != target.length:
+        raise IndexError(f"cannot fit jagged slice with length {n} into array with length {target.length}")
+    err, tooffsets, tocarry = nplike.mask_getitem(starts, stops, target.content.length, moffsets, mask.raw)
+    if err.str is not None:
+        raise IndexError(layout.backend.format_kernel_error(err))
+    total = nplike.item(tooffsets, n)
+    return C.ListOffsetArray(tooffsets, target.content._carry(tocarry[:total], True), layout.backend)
 
 
 # ---- axes ------------------------------------------------------------------------

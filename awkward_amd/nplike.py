@@ -6,8 +6,8 @@ It covers the subset of the reference's ``NumpyLike`` protocol
 (``src/awkward/_nplikes/numpy_like.py:113-552``) that the jagged-array path
 uses.  Buffers are torch tensors because PyTorch-ROCm owns device memory and
 streams here; the data-parallel work of this path (gathers, boolean-mask
-compaction, list lengths) runs in libawkward_hip.so kernels, and only plain
-elementwise ufuncs go through torch.
+compaction, list lengths, elementwise ufuncs) runs in libawkward_hip.so
+kernels; torch provides allocation and streams.
 """
 
 from __future__ import annotations
@@ -19,6 +19,7 @@ import torch
 
 from awkward_amd._kernels import KernelError, device_address, set_stream
 from awkward_amd._lib import lib
+from awkward_amd.elementwise import DeviceExpr, build, compile_program, leaf
 
 _TORCH = {
     np.dtype(np.bool_): torch.bool,
@@ -66,11 +67,17 @@ class TorchHip:
     name = "hip"
     known_data = True
 
-    def __init__(self, device: str | torch.device | None = None):
+    def __init__(self, device: str | torch.device | None = None, fuse: bool = True):
         self.device = torch.device(device if device is not None else "cuda")
+        # fuse=True: ufuncs return lazy DeviceExprs, evaluated in one pass
+        # when needed (or inside the jagged mask getitem); False: each ufunc
+        # writes its result at once, as the reference's nplike does
+        self.fuse = fuse
         self._take = {}
         self._b2i = None
         self._sub = None
+        self._eval = None
+        self._mask = None
 
     # ---- creation --------------------------------------------------------
     def is_own_array(self, x) -> bool:
@@ -104,9 +111,13 @@ class TorchHip:
 
     # ---- metadata ----------------------------------------------------------
     def dtype(self, x) -> np.dtype:
+        if isinstance(x, DeviceExpr):
+            return x.dtype
         return to_numpy_dtype(x.dtype)
 
     def length(self, x) -> int:
+        if isinstance(x, DeviceExpr):
+            return x.length
         return int(x.shape[0])
 
     def view(self, x, dtype):
@@ -161,37 +172,58 @@ class TorchHip:
         return torch.repeat_interleave(torch.arange(counts.shape[0], device=self.device), counts,
                                        output_size=int(total))
 
+    # ---- elementwise ------------------------------------------------------------
+    def gathered(self, data, index):
+        """data[index] as an elementwise operand: read in place by the
+        expression (a lazy carry is never copied)."""
+        if isinstance(data, DeviceExpr):
+            data = data.materialize()
+        return (data, index)
+
     def apply_ufunc(self, ufunc, *args):
-        """Elementwise numpy ufuncs on device tensors (torch's kernels)."""
-        name = ufunc.__name__
-        targs = [a if isinstance(a, torch.Tensor) else a for a in args]
-        fn = _UFUNCS.get(name)
-        if fn is None:
-            raise NotImplementedError(f"ufunc {name} on the hip backend")
-        return fn(*targs)
+        """A numpy ufunc over device columns (DeviceExprs, tensors, gathers)
+        and scalars, with numpy's types: a lazy DeviceExpr when fusing, else
+        written at once -- by the library's expression kernel either way."""
+        expr = build(self, ufunc, args)
+        return expr if self.fuse else expr.materialize()
 
+    def eval_expr(self, expr: DeviceExpr) -> torch.Tensor:
+        if self._eval is None:
+            f = self._eval = getattr(lib(), "awkward_hip_expr_eval")
+            f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
+            f.restype = KernelError
+        prog, keep = expr.program()
+        out = torch.empty(max(expr.length, 0), dtype=to_torch_dtype(expr.dtype), device=self.device)
+        set_stream()
+        _check(self._eval(out.data_ptr() if expr.length else 0, ctypes.addressof(prog), 0, expr.length))
+        del keep
+        return out
 
-_UFUNCS = {
-    "greater": torch.gt,
-    "greater_equal": torch.ge,
-    "less": torch.lt,
-    "less_equal": torch.le,
-    "equal": torch.eq,
-    "not_equal": torch.ne,
-    "logical_and": torch.logical_and,
-    "logical_or": torch.logical_or,
-    "bitwise_and": lambda a, b: a & b,
-    "bitwise_or": lambda a, b: a | b,
-    "absolute": torch.abs,
-    "negative": torch.neg,
-    "add": torch.add,
-    "subtract": torch.sub,
-    "multiply": torch.mul,
-    "true_divide": torch.true_divide,
-    "cosh": torch.cosh,
-    "cos": torch.cos,
-    "sinh": torch.sinh,
-    "sin": torch.sin,
-    "exp": torch.exp,
-    "sqrt": torch.sqrt,
-}
+    def mask_program(self, mask):
+        """The program of a boolean mask: a recorded predicate, or a load of
+        a boolean tensor."""
+        if isinstance(mask, DeviceExpr):
+            return mask.program()
+        return compile_program(leaf(mask))
+
+    def mask_getitem(self, fromstarts, fromstops, contentlen, moffsets, mask):
+        """(tooffsets, tocarry) of array[mask] for a jagged boolean mask over
+        int64 offsets `moffsets` and the indexed lists [fromstarts,
+        fromstops) (awkward_hip_mask_getitem: the reference's mask -> index
+        -> jagged_expand -> carrylen -> jagged_apply as the mask's bits and
+        one scan).  The carry's length is tooffsets[n].  Raises the reference's
+        IndexError message on a bad mask."""
+        if self._mask is None:
+            f = self._mask = getattr(lib(), "awkward_hip_mask_getitem")
+            f.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                                  ctypes.c_void_p]
+            f.restype = KernelError
+        n = moffsets.shape[0] - 1
+        prog, keep = self.mask_program(mask)
+        tooffsets = torch.empty(n + 1, dtype=torch.int64, device=self.device)
+        tocarry = torch.empty(max(self.length(mask), 1), dtype=torch.int64, device=self.device)
+        set_stream()
+        err = self._mask(tooffsets.data_ptr(), tocarry.data_ptr(), fromstarts.data_ptr(), fromstops.data_ptr(),
+                         int(contentlen), moffsets.data_ptr(), n, self.length(mask), ctypes.addressof(prog))
+        del keep
+        return err, tooffsets, tocarry

@@ -190,11 +190,33 @@ struct Error awkward_hip_reduce_argmin_shifts_float64(int64_t* toptr, int8_t* ma
    forbid hipMalloc).  awkward_ListArray64_getitem_next_range_64 with the
    look-back scan's scratch (tile counter and status words) in a file-scope
    1 MiB __device__ array instead of the stream's pool; no hipMalloc,
-   hipFree or host synchronisation.  At most 268,304,384 lists (a clear
+   hipFree or host synchronisation.  At most 1,073,217,536 lists (a clear
    error beyond).  Every call on one device shares that device's array, and
    each call clears what it uses first: calls that overlap on one device
    (concurrent streams, threads) race. */
 struct Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, int64_t* tocarry, const int64_t* fromstarts, const int64_t* fromstops, int64_t lenstarts, int64_t start, int64_t stop, int64_t step);
+
+/* ---- elementwise expressions -------------------------------------------------
+   A program (struct ExprProgram of awkward_amd/csrc/expr.h: up to 8 leaf
+   columns, each optionally gathered through an int64 index, and up to 40
+   typed ops over 10 registers) is a chain of numpy ufuncs with the operand
+   types already resolved by numpy's rules (array_module.py:254-332 applies
+   them one at a time).  The program pointer is a HOST pointer; the leaves
+   are device pointers.
+   expr_eval: out[i] = program(start + i), 0 <= i < n, in the program's
+     out_type.
+   mask_getitem: array[mask] for a jagged boolean mask program of masklen
+     elements over int64 offsets moffsets (n lists): the program's bits
+     (one word per 64 elements, in the stream's scratch), then the result's
+     offsets (n + 1, from 0) and
+     its carry into the content of the lists [fromstarts[i], fromstops[i]),
+     with awkward_ListArray_getitem_jagged_apply's checks and messages --
+     the reference's _normalise_item_bool_to_int + jagged_expand + carrylen
+     + jagged_apply sequence in one scan.  tocarry must hold the mask's
+     true count (an upper bound such as its element count will do); the
+     count is tooffsets[n]. */
+struct Error awkward_hip_expr_eval(void* out, const void* program, int64_t start, int64_t n);
+struct Error awkward_hip_mask_getitem(int64_t* tooffsets, int64_t* tocarry, const int64_t* fromstarts, const int64_t* fromstops, int64_t contentlen, const int64_t* moffsets, int64_t n, int64_t masklen, const void* program);
 
 /* ---- gathers and small host-layer steps ------------------------------------
    take_<bytes>: out[i] = data[index[i]] (NumpyArray._carry, numpyarray.py:367-373).

@@ -87,6 +87,9 @@ class NumpyNplike:
     def parents(self, offsets, total):
         return np.repeat(np.arange(len(offsets) - 1), np.diff(offsets))
 
+    def gathered(self, data, index):
+        return data[index]
+
     def apply_ufunc(self, ufunc, *args):
         return ufunc(*args)
 
