@@ -23,7 +23,8 @@
 // The program is uniform across the grid (a kernel argument), so the op
 // dispatch is scalar control flow; each op is dispatched once for all U
 // element slots of a thread; every leaf is loaded before the first op, so
-// all of a thread's loads are in flight together.
+// all of a thread's loads are in flight together.  Operands 0-7 name the
+// leaves, 8-15 the temporaries that op results go to.
 #pragma once
 
 #include <cmath>
@@ -81,13 +82,13 @@ enum ExprCode : uint8_t {
 
 constexpr int kExprMaxLeaves = 8;
 constexpr int kExprMaxOps = 40;
-constexpr int kExprRegs = 10;  // registers 0 .. nleaves - 1 hold the leaves
+constexpr int kExprRegs = 16;  // registers 0 .. nleaves - 1 start with the leaves
 
 struct ExprLeaf {
   const void* data;
   const int64_t* index;  // nullptr: data[i]; else data[index[i]]
   int32_t type;
-  int32_t pad;
+  int32_t leg;  // the pair kernel's leaves: which element of the pair (0 or 1) the leaf reads
 };
 
 struct ExprOp {
@@ -100,14 +101,47 @@ struct ExprOp {
 
 struct ExprProgram {
   int32_t nleaves, nops, out, out_type;
-  int32_t wide, pad;  // wide: 64-bit registers (some type of the program needs 64 bits)
+  int32_t wide;   // 64-bit registers (some type of the program needs 64 bits)
+  int32_t nregs;  // registers the program uses (every register number is below it)
   ExprLeaf leaf[kExprMaxLeaves];
   ExprOp op[kExprMaxOps];
 };
 
 inline bool expr_program_ok(const ExprProgram& p) {
-  return p.nleaves >= 0 && p.nleaves <= kExprMaxLeaves && p.nops >= 0 && p.nops <= kExprMaxOps && p.out >= 0 &&
-         p.out < kExprRegs && p.out_type >= kTBool && p.out_type <= kTFloat64;
+  if (!(p.nleaves >= 0 && p.nleaves <= kExprMaxLeaves && p.nops >= 0 && p.nops <= kExprMaxOps &&
+        p.nregs >= 1 && p.nregs <= kExprRegs && p.nleaves <= p.nregs && p.out >= 0 && p.out < p.nregs &&
+        p.out_type >= kTBool && p.out_type <= kTFloat64))
+    return false;
+  for (int k = 0; k < p.nops; k++) {
+    const ExprOp& o = p.op[k];
+    if (o.dst >= p.nregs || o.a >= p.nregs || o.b >= p.nregs || o.code > kXMin || o.type > kTFloat64) return false;
+  }
+  return true;
+}
+
+// Element slots per thread: the register file is 32 VGPRs (the widest
+// register tuple), so U slots leave 32 / (U * words) registers -- 4 slots
+// of 32-bit registers when the program needs at most 8, else 2 (64-bit
+// registers: 2, else 1).
+inline int expr_slots(const ExprProgram& p) {
+  const bool small = p.nregs <= 8;
+  return p.wide ? (small ? 2 : 1) : (small ? 4 : 2);
+}
+
+// Calls f(XKind<W, U>{}) for the program's register width and slots.
+template <typename W, int U>
+struct XKind {
+  using Word = W;
+  static constexpr int slots = U;
+};
+template <class F>
+inline void expr_dispatch(const ExprProgram& p, F f) {
+  switch (expr_slots(p) + (p.wide ? 8 : 0)) {
+    case 4: f(XKind<uint32_t, 4>{}); break;
+    case 2: f(XKind<uint32_t, 2>{}); break;
+    case 10: f(XKind<uint64_t, 2>{}); break;
+    default: f(XKind<uint64_t, 1>{}); break;
+  }
 }
 
 __device__ inline bool x_is_signed(int t) { return t == kTInt8 || t == kTInt16 || t == kTInt32 || t == kTInt64; }
@@ -366,76 +400,110 @@ __device__ inline XV<U, W> x_op_v(const ExprOp& o, const XV<U, W>& a, const XV<U
   }
 }
 
-// The register file: named vectors, each case of the switches below naming
-// one, so the file stays in VGPRs (an array indexed at run time would live
-// in scratch memory).
+// The register file: one vector of 32 VGPRs -- the widest register tuple --
+// holding 32 / (U * words) registers of U slots, indexed at run time by the
+// op's uniform register number, which the compiler turns into VGPR-indexed
+// moves (s_set_gpr_idx) rather than a select over every register or a spill
+// to scratch.  (Two files, or one wider than 32 VGPRs, lose that: the
+// compiler then copies or selects whole files.)  The file is passed and
+// returned as a value -- whole-vector extracts and inserts -- so its address
+// is never taken.
 template <int U, typename W>
-struct XRegs {
-  XV<U, W> r0, r1, r2, r3, r4, r5, r6, r7, r8, r9;
+struct XFile {
+  static constexpr int elems = 32 / (sizeof(W) / 4);
+  static constexpr int regs = elems / U;
+  typedef W vec __attribute__((ext_vector_type(elems)));
 };
-static_assert(kExprRegs == 10, "XRegs names kExprRegs registers");
+template <int U, typename W>
+using XFileV = typename XFile<U, W>::vec;
 
 template <int U, typename W>
-__device__ inline void x_get(const XRegs<U, W>& r, int k, XV<U, W>& out) {
-  switch (k) {
-    case 0: out.v = r.r0.v; break;
-    case 1: out.v = r.r1.v; break;
-    case 2: out.v = r.r2.v; break;
-    case 3: out.v = r.r3.v; break;
-    case 4: out.v = r.r4.v; break;
-    case 5: out.v = r.r5.v; break;
-    case 6: out.v = r.r6.v; break;
-    case 7: out.v = r.r7.v; break;
-    case 8: out.v = r.r8.v; break;
-    default: out.v = r.r9.v; break;
-  }
+__device__ inline XV<U, W> x_get(XFileV<U, W> f, int r) {
+  const int base = (r & (XFile<U, W>::regs - 1)) * U;
+  XV<U, W> out;
+#pragma unroll
+  for (int u = 0; u < U; u++) out.v[u] = f[base + u];
+  return out;
 }
 
 template <int U, typename W>
-__device__ inline void x_set(XRegs<U, W>& r, int k, const XV<U, W>& v) {
-  switch (k) {
-    case 0: r.r0.v = v.v; break;
-    case 1: r.r1.v = v.v; break;
-    case 2: r.r2.v = v.v; break;
-    case 3: r.r3.v = v.v; break;
-    case 4: r.r4.v = v.v; break;
-    case 5: r.r5.v = v.v; break;
-    case 6: r.r6.v = v.v; break;
-    case 7: r.r7.v = v.v; break;
-    case 8: r.r8.v = v.v; break;
-    default: r.r9.v = v.v; break;
+__device__ inline XFileV<U, W> x_set(XFileV<U, W> f, int r, XV<U, W> val) {
+  const int base = (r & (XFile<U, W>::regs - 1)) * U;
+#pragma unroll
+  for (int u = 0; u < U; u++) f[base + u] = val.v[u];
+  return f;
+}
+
+// Runs the program on a file whose first registers hold the leaves (the host
+// picks U so that every register number of the program is in the file).
+// Op k of the program read as four dwords (one scalar load) and unpacked: a
+// field-by-field read of the bytes at offsets 5 and 6 is not dword aligned
+// and would become a vector memory load per op.
+__device__ inline ExprOp x_fetch(const ExprProgram& p, int k) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(&p.op[k]);
+  const uint32_t w0 = w[0], w1 = w[1];
+  ExprOp o;
+  o.code = (uint8_t)w0;
+  o.type = (uint8_t)(w0 >> 8);
+  o.dst = (uint8_t)(w0 >> 16);
+  o.a = (uint8_t)(w0 >> 24);
+  o.b = (uint8_t)w1;
+  o.to = (uint8_t)(w1 >> 8);
+  o.bimm = (uint8_t)(w1 >> 16);
+  o.pad1 = 0;
+  o.imm = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+  return o;
+}
+static_assert(sizeof(ExprOp) == 16, "ExprOp is four dwords");
+
+template <int U, typename W>
+__device__ inline void expr_run(const ExprProgram& p, XFileV<U, W> f, W (&out)[U]) {
+  for (int k = 0; k < p.nops; k++) {
+    const ExprOp o = x_fetch(p, k);
+    const XV<U, W> a = x_get<U, W>(f, o.a);
+    XV<U, W> b;
+    if (o.bimm)
+      b.v = (W)o.imm;
+    else
+      b = x_get<U, W>(f, o.b);
+    f = x_set<U, W>(f, o.dst, x_op_v<U, W>(o, a, b));
   }
+  const XV<U, W> res = x_get<U, W>(f, p.out);
+#pragma unroll
+  for (int u = 0; u < U; u++) out[u] = res.v[u];
 }
 
 // Evaluates the program at U element positions (pos[u] < 0: slot unused),
 // with registers of width W (uint32_t: the program's `wide` is 0).
 template <int U, typename W>
 __device__ inline void expr_eval(const ExprProgram& p, const int64_t (&pos)[U], W (&out)[U]) {
-  XV<U, W> l[kExprMaxLeaves];
+  XFileV<U, W> l = 0;
 #pragma unroll
-  for (int k = 0; k < kExprMaxLeaves; k++) {
-    l[k].v = 0;
+  for (int k = 0; k < kExprMaxLeaves; k++)
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (k < p.nleaves && pos[u] >= 0) l[k].v[u] = x_load<W>(p.leaf[k], pos[u]);
-  }
-  XV<U, W> z;
-  z.v = 0;
-  XRegs<U, W> r{l[0], l[1], l[2], l[3], l[4], l[5], l[6], l[7], z, z};
-  for (int k = 0; k < p.nops; k++) {
-    const ExprOp o = p.op[k];
-    XV<U, W> a, b;
-    x_get<U, W>(r, o.a, a);
-    if (o.bimm)
-      b.v = (W)o.imm;
-    else
-      x_get<U, W>(r, o.b, b);
-    x_set<U, W>(r, o.dst, x_op_v<U, W>(o, a, b));
-  }
-  XV<U, W> res;
-  x_get<U, W>(r, p.out, res);
+      if (k < p.nleaves && pos[u] >= 0) l[k * U + u] = x_load<W>(p.leaf[k], pos[u]);
+  expr_run<U, W>(p, l, out);
+}
+
+// The same over pairs: leaf k reads element pos0[u] (leg 0) or pos1[u]
+// (leg 1) of its column.
+template <int U, typename W>
+__device__ inline void expr_eval_pairs(const ExprProgram& p, const int64_t (&pos0)[U], const int64_t (&pos1)[U],
+                                       W (&out)[U]) {
+  XFileV<U, W> l = 0;
 #pragma unroll
-  for (int u = 0; u < U; u++) out[u] = res.v[u];
+  for (int k = 0; k < kExprMaxLeaves; k++) {
+    if (k < p.nleaves) {
+      const bool second = p.leaf[k].leg != 0;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t q = second ? pos1[u] : pos0[u];
+        if (q >= 0) l[k * U + u] = x_load<W>(p.leaf[k], q);
+      }
+    }
+  }
+  expr_run<U, W>(p, l, out);
 }
 
 // Stores a canonical register as element i of an array of type t.

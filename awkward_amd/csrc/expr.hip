@@ -33,17 +33,11 @@ __global__ void fill_first_i64(int64_t* p) { *p = 0; }
 
 constexpr int kEvalBlock = 256;
 
-// U elements per thread per program pass: 4 with 32-bit registers, 2 with
-// 64-bit ones (the register file's VGPRs bound the waves in flight)
-template <typename W>
-constexpr int eval_slots() {
-  return sizeof(W) == 4 ? 4 : 2;
-}
-
-template <typename W>
+// U elements per thread per program pass (expr_slots: the register file
+// holds 32 VGPRs)
+template <typename W, int U>
 __global__ __launch_bounds__(kEvalBlock) void expr_eval_kernel(ExprProgram p, void* __restrict__ out, int64_t start,
                                                                int64_t n) {
-  constexpr int U = eval_slots<W>();
   const int64_t stride = (int64_t)gridDim.x * kEvalBlock * U;
   for (int64_t base = (int64_t)blockIdx.x * kEvalBlock * U; base < n; base += stride) {
     int64_t pos[U];
@@ -61,12 +55,11 @@ __global__ __launch_bounds__(kEvalBlock) void expr_eval_kernel(ExprProgram p, vo
 }
 
 inline void launch_eval(const ExprProgram& p, void* out, int64_t start, int64_t n) {
-  if (p.wide)
-    expr_eval_kernel<uint64_t><<<grid_for(n, kEvalBlock, eval_slots<uint64_t>()), kEvalBlock, 0, stream()>>>(
-        p, out, start, n);
-  else
-    expr_eval_kernel<uint32_t><<<grid_for(n, kEvalBlock, eval_slots<uint32_t>()), kEvalBlock, 0, stream()>>>(
-        p, out, start, n);
+  expr_dispatch(p, [&](auto kind) {
+    using W = typename decltype(kind)::Word;
+    constexpr int U = decltype(kind)::slots;
+    expr_eval_kernel<W, U><<<grid_for(n, kEvalBlock, U), kEvalBlock, 0, stream()>>>(p, out, start, n);
+  });
 }
 
 // jagged_apply's messages (the same codes and order: per list, the checks
@@ -83,10 +76,9 @@ const char* const kMaskMsgs[] = {
 // The predicate as bits: bit (i & 63) of word i >> 6 is program(i) for
 // 0 <= i < n (the words past n are not written).  Each wave evaluates 64
 // consecutive elements per slot and stores their ballot as one word.
-template <typename W>
+template <typename W, int U>
 __global__ __launch_bounds__(kEvalBlock) void expr_bits_kernel(ExprProgram p, uint64_t* __restrict__ bits,
                                                                int64_t n) {
-  constexpr int U = eval_slots<W>();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int64_t stride = (int64_t)gridDim.x * kEvalBlock * U;
@@ -285,12 +277,11 @@ Error awkward_hip_mask_getitem(int64_t* tooffsets, int64_t* tocarry, const int64
   const uint32_t inv = next_invocation("awkward_ListArray_getitem_jagged_apply", kMaskMsgs);
   const ScanScratch s = scan_scratch_at(base + bits_bytes);
   if (elements > 0) {
-    if (p.wide)
-      expr_bits_kernel<uint64_t><<<grid_for(elements, kEvalBlock, eval_slots<uint64_t>()), kEvalBlock, 0, st>>>(
-          p, bits, elements);
-    else
-      expr_bits_kernel<uint32_t><<<grid_for(elements, kEvalBlock, eval_slots<uint32_t>()), kEvalBlock, 0, st>>>(
-          p, bits, elements);
+    expr_dispatch(p, [&](auto kind) {
+      using W = typename decltype(kind)::Word;
+      constexpr int U = decltype(kind)::slots;
+      expr_bits_kernel<W, U><<<grid_for(elements, kEvalBlock, U), kEvalBlock, 0, st>>>(p, bits, elements);
+    });
   }
   scan_prep_kernel<0><<<(unsigned)ceil_div(ntiles + 1, 256), 256, 0, st>>>(s, ntiles);
   mask_bits_getitem_kernel<<<(unsigned)ntiles, kMGBlock, 0, st>>>(bits, tooffsets, tocarry, fromstarts, fromstops,

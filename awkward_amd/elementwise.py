@@ -24,6 +24,7 @@ answer; arithmetic with one raises numpy's OverflowError.
 from __future__ import annotations
 
 import ctypes
+import heapq
 
 import numpy as np
 
@@ -52,12 +53,12 @@ COMPARISONS = {GT, GE, LT, LE, EQ, NE}
 
 MAX_LEAVES = 8
 MAX_OPS = 40
-MAX_REGS = 10
+MAX_REGS = 16
 
 
 class _Leaf(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("index", ctypes.c_void_p), ("type", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("leg", ctypes.c_int32)]
 
 
 class _Op(ctypes.Structure):
@@ -70,7 +71,7 @@ class Program(ctypes.Structure):
     """expr.h's ExprProgram."""
 
     _fields_ = [("nleaves", ctypes.c_int32), ("nops", ctypes.c_int32), ("out", ctypes.c_int32),
-                ("out_type", ctypes.c_int32), ("wide", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("out_type", ctypes.c_int32), ("wide", ctypes.c_int32), ("nregs", ctypes.c_int32),
                 ("leaf", _Leaf * MAX_LEAVES), ("op", _Op * MAX_OPS)]
 
 
@@ -88,25 +89,28 @@ def _bits(value, dtype: np.dtype) -> int:
 class Node:
     """One node of an expression tree (a leaf column, a constant or an op)."""
 
-    __slots__ = ("kind", "dtype", "code", "children", "data", "index", "value", "loop")
+    __slots__ = ("kind", "dtype", "code", "children", "data", "index", "value", "loop", "leg")
 
-    def __init__(self, kind, dtype, code=None, children=(), data=None, index=None, value=None, loop=None):
+    def __init__(self, kind, dtype, code=None, children=(), data=None, index=None, value=None, loop=None, leg=0):
         self.kind, self.dtype, self.code, self.children = kind, np.dtype(dtype), code, tuple(children)
-        self.data, self.index, self.value, self.loop = data, index, value, loop
+        self.data, self.index, self.value, self.loop, self.leg = data, index, value, loop, leg
 
     def key(self):
         """Identity of a leaf or constant (ops are identified by their
         children's positions when a program is compiled)."""
         if self.kind == "leaf":
             return ("leaf", self.data.data_ptr(), None if self.index is None else self.index.data_ptr(),
-                    self.dtype.str)
+                    self.dtype.str, self.leg)
         return ("const", self.value, self.dtype.str)
 
 
-def leaf(data, index=None) -> Node:
+def leaf(data, index=None, leg=0) -> Node:
+    """A column read at each element's position (through `index` when given);
+    in a pair program (awkward_hip_pair_reduce) `leg` 0 / 1 reads it at the
+    pair's first / second element."""
     from awkward_amd.nplike import to_numpy_dtype
 
-    return Node("leaf", to_numpy_dtype(data.dtype), data=data, index=index)
+    return Node("leaf", to_numpy_dtype(data.dtype), data=data, index=index, leg=leg)
 
 
 def const(value, dtype) -> Node:
@@ -293,7 +297,10 @@ def compile_program(root: Node):
         for c in (e[1][:1] if e[4] else e[1]):
             last_use[c] = i
     last_use[out_id] = len(order)
-    reg, free = {}, [r for r in range(MAX_REGS - 1, len(leaves) - 1, -1)]
+    # registers 0 .. nleaves - 1 start with the leaves; the lowest free
+    # register takes each result, so `nregs` (and with it the device's
+    # register file) stays small
+    reg, free = {}, list(range(len(leaves), MAX_REGS))
     prog = Program()
     keep = []
     for r, i in enumerate(leaves):
@@ -302,18 +309,19 @@ def compile_program(root: Node):
         prog.leaf[r].data = n.data.data_ptr()
         prog.leaf[r].index = 0 if n.index is None else n.index.data_ptr()
         prog.leaf[r].type = TYPE_CODE[n.dtype]
+        prog.leaf[r].leg = n.leg
         keep.extend([n.data, n.index])
     nops = 0
     for i, (n, kids, code, side, bimm) in enumerate(order):
         if n.kind == "leaf" or (n.kind == "const" and i not in last_use):
             continue  # leaves sit in their registers; constants used only as immediates need none
-        # operands whose last use is this op free their registers first
+        # operands whose last use is this op free their registers for its result
         for c in set(kids[:1] if bimm else kids):
             if last_use.get(c) == i:
-                free.append(reg[c])
+                heapq.heappush(free, reg[c])
         if not free:
             raise OverflowError("expression needs too many registers")
-        dst = free.pop()
+        dst = heapq.heappop(free)
         reg[i] = dst
         if nops >= MAX_OPS:
             raise OverflowError("expression has too many operations")
@@ -341,4 +349,5 @@ def compile_program(root: Node):
                         for e in order))
     prog.out = reg[out_id]
     prog.out_type = TYPE_CODE[root.dtype]
+    prog.nregs = max([prog.out, len(leaves) - 1] + [prog.op[k].dst for k in range(nops)]) + 1
     return prog, keep

@@ -10,7 +10,9 @@ in one kernel (``awkward_hip_event_pipeline``) with the same results as the
 per-kernel path; ``pair_max`` evaluates ak.max / ak.argmax over
 ``ak.combinations(ev, 2)`` of the pair expression
 ``2*pt1*pt2*(cosh(eta1-eta2) - cos(phi1-phi2))`` without materialising the
-pairs (``awkward_hip_pair_max``).  Neither reads anything back to the host,
+pairs (``awkward_hip_pair_max``), and ``PairReduce`` does the same for any
+elementwise expression of the pairs' fields, given as a program
+(``awkward_hip_pair_reduce``).  Neither reads anything back to the host,
 so ``Graph`` can capture them (with torch.cuda.CUDAGraph, which is a hipGraph
 on ROCm) and replay the whole sequence with one launch.
 """
@@ -144,6 +146,43 @@ class PairMax:
         o = self.out
         _check(self._f(device_address(self.offsets), self.n, *[c.data_ptr() for c in self.cols], o["max"].data_ptr(),
                        o["argmax"].data_ptr(), o["mask"].data_ptr(), float("-inf")))
+        return o
+
+
+class PairReduce:
+    """ak.max / ak.min / ak.argmax / ak.argmin / ak.sum (axis=-1) of one
+    elementwise expression over the pairs of ``ak.combinations(ev, 2)``,
+    from a program whose leaves read leg 0 or leg 1 of each pair
+    (``awkward_hip_pair_reduce``): the pairs and the expression's values are
+    never written.  ``expr`` is an ``elementwise.DeviceExpr`` of float32 or
+    float64; ``offsets`` are the lists' int64 offsets into its columns."""
+
+    _SLOTS = ("max", "min", "sum", "argmax", "argmin", "mask")
+
+    def __init__(self, offsets, expr, outputs=("max", "argmax")):
+        self.offsets = offsets
+        self.n = offsets.shape[0] - 1
+        self.program, self._keep = expr.program()
+        dt = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}.get(np.dtype(expr.dtype))
+        if dt is None:
+            raise NotImplementedError(f"pair reductions of {expr.dtype} expressions")
+        dev = offsets.device
+        self.out = {}
+        for name in outputs:
+            if name not in self._SLOTS[:5]:
+                raise ValueError(f"unknown pair reduction {name!r}")
+            self.out[name] = torch.empty(self.n, dtype=dt if name in ("max", "min", "sum") else torch.int64,
+                                         device=dev)
+        if any(o != "sum" for o in outputs):
+            self.out["mask"] = torch.empty(self.n, dtype=torch.int8, device=dev)
+        P, I64 = ctypes.c_void_p, ctypes.c_int64
+        self._f = _fn("awkward_hip_pair_reduce", [P, I64, P, P, P, P, P, P, P])
+
+    def run(self):
+        set_stream()
+        o = self.out
+        _check(self._f(device_address(self.offsets), self.n, ctypes.addressof(self.program),
+                       *[o[k].data_ptr() if k in o else 0 for k in self._SLOTS]))
         return o
 
 

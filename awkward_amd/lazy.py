@@ -38,6 +38,19 @@ _NP_CMP = {"gt": np.greater, "ge": np.greater_equal, "lt": np.less, "le": np.les
            "ne": np.not_equal}
 _NP_BIN = {"add": np.add, "sub": np.subtract, "mul": np.multiply, "div": np.true_divide, "and": np.bitwise_and,
            "or": np.bitwise_or}
+_NP_UN = {"abs": np.absolute, "neg": np.negative}
+
+
+def _ufunc(op, table):
+    """The numpy ufunc of a Binary / Unary node's op: the operator names
+    above, or a numpy ufunc's own name (nodes recorded by __array_ufunc__)."""
+    f = table.get(op)
+    if f is None:
+        f = getattr(np, op, None)
+        if not isinstance(f, np.ufunc):
+            raise NotImplementedError(f"lazy op {op!r}")
+    return f
+
 
 
 # ---- nodes ---------------------------------------------------------------------------
@@ -172,6 +185,21 @@ class LazyArray:
     def __neg__(self):
         return self._wrap(Unary(self.node, op="neg"))
 
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        """numpy ufuncs on lazy arrays record a node (np.cosh(lz.eta), ...)."""
+        if method != "__call__" or kwargs or ufunc.nout != 1:
+            return NotImplemented
+        name = ufunc.__name__
+        nodes = [_node(x) for x in inputs]
+        if len(nodes) == 1:
+            return self._wrap(Unary(nodes[0], op=name))
+        if len(nodes) == 2:
+            cmp = {v.__name__: k for k, v in _NP_CMP.items()}
+            if name in cmp:
+                return self._wrap(Compare(nodes[0], nodes[1], op=cmp[name]))
+            return self._wrap(Binary(nodes[0], nodes[1], op=name))
+        return NotImplemented
+
     __hash__ = None
 
     def __bool__(self):
@@ -277,9 +305,9 @@ def _eager(node, memo):
     elif isinstance(node, Compare):
         out = _NP_CMP[p["op"]](vals[0], vals[1])
     elif isinstance(node, Binary):
-        out = _NP_BIN[p["op"]](vals[0], vals[1])
+        out = _ufunc(p["op"], _NP_BIN)(vals[0], vals[1])
     elif isinstance(node, Unary):
-        out = abs(vals[0]) if p["op"] == "abs" else -vals[0]
+        out = _ufunc(p["op"], _NP_UN)(vals[0])
     elif isinstance(node, Reduce):
         out = getattr(ak, p["op"])(vals[0], axis=-1) if p["op"] != "num" else ak.num(vals[0], axis=1)
     elif isinstance(node, Combinations):
@@ -362,16 +390,137 @@ def _record_dtypes(array):
     return out
 
 
+# ---- fusion: reductions over the pairs of COMBINATIONS(n=2) ---------------------------------
+_PAIR_OPS = ("max", "min", "argmax", "argmin", "sum")
+
+
+class _NoPair(Exception):
+    pass
+
+
+def _list_source(node):
+    """The ListOffsetArray layout of a hip array (fused backend, int64
+    offsets) that `node` -- an Input -- holds, else None."""
+    if not isinstance(node, Input):
+        return None
+    array = node.params["array"]
+    layout = getattr(array, "layout", None)
+    be = getattr(array, "backend", None)
+    if not isinstance(layout, C.ListOffsetArray) or not getattr(be, "fused", False) or be.nplike.name != "hip":
+        return None
+    if be.nplike.dtype(layout.offsets) != np.int64:
+        return None
+    return layout
+
+
+def _pair_legs(comb):
+    """The two leg names of a COMBINATIONS(n=2, replacement=False) node whose
+    input the pair kernel reads, else None."""
+    p = comb.params
+    if p["n"] != 2 or p["replacement"] or _list_source(comb.inputs[0]) is None:
+        return None
+    return tuple(p["fields"]) if p["fields"] is not None else ("0", "1")
+
+
+def _pair_combination(tree):
+    """The one COMBINATIONS node whose legs the elementwise `tree` reads."""
+    found = set()
+
+    def walk(n):
+        if isinstance(n, Combinations):
+            found.add(n)
+        elif isinstance(n, (Field, Binary, Unary, Compare)):
+            for c in n.inputs:
+                walk(c)
+        elif not isinstance(n, Constant):
+            raise _NoPair
+
+    walk(tree)
+    if len(found) != 1:
+        raise _NoPair
+    return found.pop()
+
+
+def _pair_expr(node, comb, legs, content, nplike, memo):
+    """An elementwise tree over the legs of `comb` as a DeviceExpr whose
+    leaves read leg 0 or 1 of each pair (Python scalars stay scalars, so the
+    types are the eager ufuncs' own); _NoPair for anything else."""
+    import torch
+
+    from awkward_amd import elementwise as E
+
+    if node.id in memo:
+        return memo[node.id]
+    if isinstance(node, Constant):
+        out = node.params["value"]
+    elif isinstance(node, Field):
+        x, name = node.inputs[0], node.params["name"]
+        if x is comb and isinstance(content, C.NumpyArray):  # pairs["0"] of a list of numbers
+            leg, col = name, content
+        elif isinstance(x, Field) and x.inputs[0] is comb and isinstance(content, C.RecordArray):
+            leg = x.params["name"]
+            if name not in content.field_names():
+                raise _NoPair
+            col = content._getitem_field(name)
+        else:
+            raise _NoPair
+        if leg not in legs or not isinstance(col, C.NumpyArray) or not isinstance(col.raw, torch.Tensor):
+            raise _NoPair
+        out = E.DeviceExpr(nplike, E.leaf(col.raw, leg=legs.index(leg)), 1)
+    elif isinstance(node, (Binary, Unary, Compare)):
+        args = [_pair_expr(c, comb, legs, content, nplike, memo) for c in node.inputs]
+        if not any(isinstance(a, E.DeviceExpr) for a in args):
+            raise _NoPair
+        op = node.params["op"]
+        uf = _NP_CMP[op] if isinstance(node, Compare) else _ufunc(op, _NP_BIN if isinstance(node, Binary) else _NP_UN)
+        try:
+            out = E.build(nplike, uf, args)
+        except (NotImplementedError, TypeError, OverflowError):
+            raise _NoPair from None
+    else:
+        raise _NoPair
+    memo[node.id] = out
+    return out
+
+
+def _match_pair(node):
+    """(COMBINATIONS node, expression node, op, DeviceExpr) when `node` is a
+    reduction at axis=-1 of a float expression over the pairs, else None."""
+    from awkward_amd import elementwise as E
+
+    if not isinstance(node, Reduce) or node.params["op"] not in _PAIR_OPS:
+        return None
+    tree = node.inputs[0]
+    try:
+        comb = _pair_combination(tree)
+        legs = _pair_legs(comb)
+        if legs is None:
+            return None
+        layout = _list_source(comb.inputs[0])
+        expr = _pair_expr(tree, comb, legs, layout.content, layout.backend.nplike, {})
+    except _NoPair:
+        return None
+    if not isinstance(expr, E.DeviceExpr) or np.dtype(expr.dtype) not in (np.float32, np.float64):
+        return None
+    return comb, tree, node.params["op"], expr
+
+
 class Plan:
     """The execution plan of one or more lazy outputs: fused event-pipeline
-    launches for the reductions that match, eager host-layer evaluation for
-    the rest."""
+    launches for the reductions that match, one pair-kernel launch per
+    expression over the pairs of COMBINATIONS(n=2) for the reductions of it
+    (``awkward_hip_pair_reduce``), the carries of COMBINATIONS outputs from
+    the offsets alone, and eager host-layer evaluation for the rest."""
 
     def __init__(self, outputs, fuse=True):
         self.outputs = list(outputs)
         self.groups = {}  # (source id, terms, slice, reduce field) -> group
         self.fused_nodes = {}
         self._pipelines = {}
+        self.pair_groups = {}  # (combinations id, expression id) -> group
+        self.combos = {}  # combinations id -> node
+        self._pairs = {}
+        self._combos = {}
         if not fuse:
             return
         f32 = np.dtype(np.float32)
@@ -379,6 +528,7 @@ class Plan:
         for out in self.outputs:
             m = _match(out.node)
             if m is None:
+                self._add_other(out)
                 continue
             source, terms, sl, name, field = m
             dtypes = _record_dtypes(source.params["array"])
@@ -402,6 +552,52 @@ class Plan:
                 red = next((f for f, dt in dtypes.items() if dt == f32), None)
             if red is not None:
                 self._add(out, source, terms, sl, red, "num")
+
+    def _add_other(self, out):
+        node = out.node
+        m = _match_pair(node)
+        if m is not None:
+            comb, tree, op, expr = m
+            key = ("pairs", comb.id, tree.id)
+            g = self.pair_groups.setdefault(key, {"comb": comb, "expr": expr, "ops": set(), "nodes": []})
+            g["ops"].add(op)
+            g["nodes"].append((node.id, op))
+            self.fused_nodes[node.id] = key
+        elif isinstance(node, Combinations) and _list_source(node.inputs[0]) is not None:
+            key = ("combinations", node.id)
+            self.combos[key] = node
+            self.fused_nodes[node.id] = key
+
+    def _pair_reduce(self, key):
+        from awkward_amd.fused import PairReduce
+
+        p = self._pairs.get(key)
+        if p is None:
+            g = self.pair_groups[key]
+            offsets = _list_source(g["comb"].inputs[0]).offsets
+            p = PairReduce(offsets, g["expr"], outputs=tuple(o for o in _PAIR_OPS if o in g["ops"]))
+            self._pairs[key] = p
+        return p
+
+    def _combinations(self, key):
+        from awkward_amd.fused import Combinations as Decode
+
+        d = self._combos.get(key)
+        if d is None:
+            node = self.combos[key]
+            d = Decode(node.inputs[0].params["array"], node.params["n"], node.params["replacement"])
+            self._combos[key] = d
+        return d
+
+    def _combinations_array(self, key):
+        from awkward_amd.highlevel import Array
+
+        node, d = self.combos[key], self._combinations(key)
+        layout = _list_source(node.inputs[0])
+        be = layout.backend
+        contents = [layout.content._carry(c[:d.total], True) for c in d.carry]
+        record = C.RecordArray(contents, node.params["fields"], d.total, be)
+        return Array(C.ListOffsetArray(d.tooffsets, record, be))
 
     def _add(self, out, source, terms, sl, red, name):
         key = (source.id, terms, (sl.start, sl.stop, sl.step), red)
@@ -440,11 +636,24 @@ class Plan:
                 "max": lambda o=o, be=be: Array(C.ByteMaskedArray(o["max_mask"], C.NumpyArray(o["max"], be), False,
                                                                   be)),
             }
+        for key in self.pair_groups:
+            o = self._pair_reduce(key).run()
+            be = _list_source(self.pair_groups[key]["comb"].inputs[0]).backend
+            masked = {k: lambda o=o, be=be, k=k: Array(C.ByteMaskedArray(o["mask"], C.NumpyArray(o[k], be), False, be))
+                      for k in ("max", "min", "argmax", "argmin")}
+            results[key] = dict(masked, sum=lambda o=o, be=be: Array(C.NumpyArray(o["sum"], be)))
+        for key in self.combos:
+            self._combinations(key).run()
         out = []
         for lz in self.outputs:
             key = self.fused_nodes.get(lz.node.id)
             if key is None:
                 out.append(_eager(lz.node, memo))
+            elif key in self.combos:
+                out.append(self._combinations_array(key))
+            elif key in self.pair_groups:
+                name = dict(self.pair_groups[key]["nodes"])[lz.node.id]
+                out.append(results[key][name]())
             else:
                 name = dict(self.groups[key]["nodes"])[lz.node.id]
                 out.append(results[key][name]())
@@ -455,10 +664,17 @@ class Plan:
         reruns them on the same inputs; the outputs' buffers are reused."""
         from awkward_amd.fused import Graph
 
-        if not self.groups or any(lz.node.id not in self.fused_nodes for lz in self.outputs):
+        from awkward_amd.fused import Concurrent
+
+        if not self.fused_nodes or any(lz.node.id not in self.fused_nodes for lz in self.outputs):
             raise ValueError("only plans whose every output is fused can be captured")
-        pipes = [self._pipeline(key) for key in self.groups]
-        return Graph(lambda: [p.run() for p in pipes])
+        reductions = [self._pipeline(key) for key in self.groups] + [self._pair_reduce(k) for k in self.pair_groups]
+        decodes = [self._combinations(key) for key in self.combos]
+        if reductions and decodes:
+            # the reductions (latency-bound) and the carries (write-bound)
+            # are independent: two branches of the graph
+            return Graph(Concurrent(lambda: [d.run() for d in decodes], lambda: [p.run() for p in reductions]))
+        return Graph(lambda: [p.run() for p in reductions + decodes])
 
     def stats(self):
         """Counts in the spirit of the reference's fusion_stats: graph nodes,
@@ -476,7 +692,8 @@ class Plan:
         for lz in self.outputs:
             walk(lz.node)
         fused = len(self.fused_nodes)
-        return {"nodes": len(seen), "fused_reductions": fused, "fused_launches": len(self.groups),
+        return {"nodes": len(seen), "fused_reductions": fused - len(self.combos),
+                "fused_launches": len(self.groups) + len(self.pair_groups) + len(self.combos),
                 "eager_outputs": len(self.outputs) - fused}
 
 

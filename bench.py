@@ -155,15 +155,47 @@ def pairs_step(ak, np, ev):
     p0, p1 = pairs["0"], pairs["1"]
     expr = 2 * p0.pt * p1.pt * (np.cosh(p0.eta - p1.eta) - np.cos(p0.phi - p1.phi))
     triples = ak.combinations(ev, 3)
-    return ak.max(expr, axis=-1), ak.argmax(expr, axis=-1), triples
+    return ak.max(expr, axis=-1), ak.argmax(expr, axis=-1), pairs, triples
+
+
+def pairs_plan(ev):
+    """The same step recorded lazily: the reductions of the pair expression
+    become one awkward_hip_pair_reduce launch, the n=2 and n=3 carries two
+    decodes -- the same four outputs as pairs_step."""
+    import numpy as np
+
+    from awkward_amd.lazy import Plan, lazy
+
+    lz = lazy(ev)
+    pairs = lz.combinations(2)
+    p0, p1 = pairs["0"], pairs["1"]
+    expr = 2 * p0.pt * p1.pt * (np.cosh(p0.eta - p1.eta) - np.cos(p0.phi - p1.phi))
+    return Plan([expr.max(), expr.argmax(), pairs, lz.combinations(3)])
+
+
+def same_outputs(torch, a, b):
+    """Bit equality of pairs_step's four outputs (masked max / argmax, two
+    combinations layouts) between two forms."""
+    for x, y in zip(a[:2], b[:2]):
+        lx, ly = x.layout, y.layout
+        if not (torch.equal(lx.mask.to(torch.int8), ly.mask.to(torch.int8)) and
+                torch.equal(lx.content.data, ly.content.data)):
+            return False
+    for x, y in zip(a[2:], b[2:]):
+        lx, ly = x.layout, y.layout
+        if not torch.equal(lx.offsets, ly.offsets):
+            return False
+        if not all(torch.equal(cx.index, cy.index) for cx, cy in zip(lx.content.contents, ly.content.contents)):
+            return False
+    return True
 
 
 def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
     """Combinations n=2 and n=3 on 20M events (Poisson(4) clipped [0, 30],
-    seed 3), the float32 pair expression, max/argmax over pairs per event."""
+    seed 3), the float32 pair expression, max/argmax over pairs per event;
+    both forms write the same outputs (the masked max and argmax, the n=2 and
+    the n=3 combinations)."""
     import numpy as np
-
-    from awkward_amd.fused import Combinations, Concurrent, Graph, PairMax
 
     off, fields = make_events(n_events, seed=3, lam=4, clip=30)
     fields = {k: fields[k] for k in ("pt", "eta", "phi")}
@@ -171,31 +203,37 @@ def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
     torch.cuda.synchronize()
     out = {"events": n_events, "particles": int(off[-1])}
     out["plain"] = time_form(torch, lambda: pairs_step(ak, np, ev), steps, 1, n_events)
-    pm = PairMax(ev)
-    tri = Combinations(ev, 3)
-    out["pairs"] = int(ak.num(ak.combinations(ev, 2)).layout.data.sum().item())
+    plan = pairs_plan(ev)
+    stats = plan.stats()
+    assert stats["eager_outputs"] == 0 and stats["fused_launches"] == 3, stats
+    fused_out = plan.run()
+    plain_out = pairs_step(ak, np, ev)
+    out["outputs_equal"] = same_outputs(torch, fused_out, plain_out)
+    out["pairs"] = int(plain_out[2].layout.offsets[-1].item())
+    del plain_out
+    pr = plan._pair_reduce(next(iter(plan.pair_groups)))
+    tri = plan._combinations(next(k for k, n in plan.combos.items() if n.params["n"] == 3))
     out["triples"] = tri.total
-
-    # pair_max (latency-bound) and the triples (write-bound) are independent:
-    # two branches of the graph
-    fused = Concurrent(tri.run, pm.run)
-
-    graph = Graph(fused)
+    graph = plan.graph()
     out["fused_graph"] = time_form(torch, graph.replay, steps, 1, n_events)
     out["speedup"] = round(out["fused_graph"]["events_per_s"] / out["plain"]["events_per_s"], 2)
-    # the triples' decode alone: write-bound, 8 * 3 * P bytes
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    tri.run()
-    s.record()
-    tri.run()
-    e.record()
-    torch.cuda.synchronize()
-    ms = s.elapsed_time(e)
-    out["combinations3"] = {"ms": round(ms, 4), "GB/s": round(tri.algorithmic_bytes() / ms / 1e6, 1),
-                            "bytes": tri.algorithmic_bytes()}
-    out["what"] = ("plain: ak.combinations(ev,2) + torch ufunc expression + ak.max/argmax + ak.combinations(ev,3); "
-                   "fused_graph: awkward_hip_pair_max and combinations_length/decode for n=3 as two concurrent "
-                   "branches of one hipGraph")
+    # the parts alone: the pair kernel (latency-bound) and the triples'
+    # decode (write-bound, 8 * 3 * P bytes)
+    for name, fn, nbytes in (("pair_reduce", pr.run, None), ("combinations3", tri.run, tri.algorithmic_bytes())):
+        fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e)
+        out[name] = {"ms": round(ms, 4)}
+        if nbytes:
+            out[name].update({"GB/s": round(nbytes / ms / 1e6, 1), "bytes": nbytes})
+    out["what"] = ("plain: ak.combinations(ev,2) + the pair expression as one expression-VM pass over the carried "
+                   "legs + ak.max/argmax + ak.combinations(ev,3); fused_graph: the same graph recorded with "
+                   "lazy(ev) -- awkward_hip_pair_reduce over the pairs in place, beside the n=2 and n=3 carries' "
+                   "decodes, as two concurrent branches of one hipGraph")
     return out
 
 
@@ -359,7 +397,7 @@ def roofline_probe(args):
     import awkward_amd as ak
     from awkward_amd.backend import HipBackend
     from awkward_amd.batch import BatchRunner
-    from awkward_amd.fused import Combinations, EventPipeline, PairMax
+    from awkward_amd.fused import EventPipeline
 
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -389,8 +427,7 @@ def roofline_probe(args):
     f3 = {k: f3[k] for k in ("pt", "eta", "phi")}
     ev3 = ak.from_offsets(off3, f3, tuned)
     pairs_step(ak, np, ev3)
-    PairMax(ev3).run()
-    Combinations(ev3, 3).run()
+    pairs_plan(ev3).run()
     torch.cuda.synchronize()
     del ev3
     batch = make_batch(max_exp=4.5)

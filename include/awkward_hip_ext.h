@@ -259,6 +259,24 @@ struct Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64
 struct Error awkward_hip_pair_max(const int64_t* offsets, int64_t n, const float* pt, const float* eta, const float* phi, float* out_max, int64_t* out_argmax, int8_t* out_mask, float identity);
 struct Error awkward_hip_pair_expr(float* out, const float* pt1, const float* eta1, const float* phi1, const float* pt2, const float* eta2, const float* phi2, int64_t n);
 
+/* ---- reductions over the pairs of each list, from an expression program ------
+   pair_reduce: ak.max / ak.min / ak.argmax / ak.argmin / ak.sum at axis=-1
+     of an elementwise expression of ak.combinations(array, 2) -- the
+     reference's awkward_ListArray_combinations_length + _combinations,
+     per-leg carries, one apply_ufunc per op and awkward_reduce_{max,min,
+     argmax,argmin,sum} (listoffsetarray.py:1152-1261, array_module.py:
+     254-332) -- without writing the pairs or the expression's values.
+     program (a HOST pointer) is an expr.h ExprProgram of float32 or
+     float64 whose leaves have leg 0 (read at the pair's first element) or
+     1 (its second); offsets are the n lists' int64 offsets, absolute
+     positions in the leaves' columns.  Pairs are taken i < j in the
+     lexicographic order of awkward_ListArray_combinations; max / min start
+     from -inf / +inf, argmax / argmin are local pair indices (first of
+     equals, -1 without pairs), sums accumulate left to right in the
+     program's type, mask is 1 for a list without pairs; any output may be
+     NULL. */
+struct Error awkward_hip_pair_reduce(const int64_t* offsets, int64_t n, const void* program, void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
+
 /* ---- batch runner: many arrays of one dtype, persistent work queues ------------
    items: device table of 64-byte rows {data, offsets, out_sum, out_min,
    out_argmin, out_mask, n, tile_begin} (reduce) or 56-byte rows {data,
