@@ -15,6 +15,7 @@
 // improvement for arg-reductions, left-to-right accumulation for sums -- so
 // the results are the elementwise form's bits.
 
+#include "combin.h"
 #include "common.h"
 #include "expr.h"
 
@@ -44,6 +45,138 @@ struct PairSlots {
   int32_t nslots;
 };
 
+// Combinations decoded by the same launch: up to two tuple sizes n (2 or 3,
+// without replacement), each with the lists' tuple offsets (the
+// combinations_length scan's tooffsets) and its n carries.
+struct CombReq {
+  int32_t nreq;
+  int32_t n[2];
+  const int64_t* tooffsets[2];
+  int64_t* carry[2][3];
+};
+
+constexpr int kTupPer = 8;                     // consecutive slots a thread decodes per chunk
+constexpr int kTupChunk = kPRLists * kTupPer;  // slots per chunk
+constexpr int kTupPad = kTupChunk + kTupChunk / kTupPer;
+__device__ inline int tup_pad(int i) { return i + (i >> 3); }  // thread t's 8 slots at t * 9: no bank conflicts
+
+// The rank-r N-tuple (lexicographic) of a list of m items, in 32 bits.
+template <int N>
+__device__ inline void decode_tuple32(int r, int m, int (&x)[N]) {
+  constexpr int TABMAX = N == 2 ? kColex2Max : kColex3Max;
+  constexpr int BITS = N == 2 ? 6 : 5;
+  constexpr int MASK = (1 << BITS) - 1;
+  if (m <= TABMAX) {
+    const int total = N == 2 ? m * (m - 1) / 2 : m * (m - 1) * (m - 2) / 6;
+    const uint32_t y = N == 2 ? kColex.pairs[total - 1 - r] : kColex.triples[total - 1 - r];
+#pragma unroll
+    for (int q = 0; q < N; q++) x[q] = m - 1 - (int)((y >> (BITS * (N - 1 - q))) & MASK);
+  } else {
+    int64_t x64[N];
+    if constexpr (N == 2) unrank2(r, m, x64);
+    else unrank3(r, m, x64);
+#pragma unroll
+    for (int q = 0; q < N; q++) x[q] = (int)x64[q];
+  }
+}
+
+// The next tuple of the list in lexicographic order; false past its last.
+template <int N>
+__device__ inline bool next_tuple(int (&x)[N], int m) {
+  if constexpr (N == 2) {
+    if (++x[1] == m) {
+      ++x[0];
+      x[1] = x[0] + 1;
+    }
+    return x[0] < m - 1;
+  } else {
+    if (++x[2] == m) {
+      if (++x[1] == m - 1) {
+        ++x[0];
+        x[1] = x[0] + 1;
+      }
+      x[2] = x[1] + 1;
+    }
+    return x[0] < m - 2;
+  }
+}
+
+// Writes the N-tuples of this workgroup's lists, in chunks of kTupChunk
+// slots: thread t decodes slots [t * 8, t * 8 + 8) of the chunk -- one
+// binary search over the group's tuple offsets and one table decode for the
+// first, then steps in lexicographic order -- into an LDS stage as element
+// positions relative to the group's first element, and the chunk is then
+// stored with each carry written 64 consecutive slots per wave instruction.
+// Group-relative values are 32-bit (the caller checks that the group's
+// elements and tuples fit).  s_tbl / s_offl: the group's tuple and element
+// offsets relative to its first; stage: N * kTupPad ints.
+template <int N>
+__device__ inline void group_tuples(int64_t* const* carry, int64_t g0, int64_t E0, const int* s_tbl,
+                                    const int* s_offl, int nl, int* stage) {
+  const int tid = threadIdx.x;
+  const int T = s_tbl[nl];
+  for (int c0 = 0; c0 < T; c0 += kTupChunk) {
+    const int cn = T - c0 < kTupChunk ? T - c0 : kTupChunk;
+    int q = c0 + tid * kTupPer;
+    const int qe = q + kTupPer < c0 + cn ? q + kTupPer : c0 + cn;
+    if (q < qe) {
+      int lo = 0, hi = nl;  // the last list e with s_tbl[e] <= q
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_tbl[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+      int e = lo;
+      int base = s_offl[e], m = s_offl[e + 1] - base;
+      int x[N];
+      decode_tuple32<N>(q - s_tbl[e], m, x);
+      for (;;) {
+        const int at = tup_pad(q - c0);
+#pragma unroll
+        for (int c = 0; c < N; c++) stage[c * kTupPad + at] = base + x[c];
+        if (++q >= qe) break;
+        if (!next_tuple<N>(x, m)) {  // on to the next list with tuples
+          do {
+            e++;
+          } while (s_tbl[e + 1] == s_tbl[e]);
+          base = s_offl[e];
+          m = s_offl[e + 1] - base;
+#pragma unroll
+          for (int c = 0; c < N; c++) x[c] = c;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < cn; i += kPRLists) {
+      const int at = tup_pad(i);
+#pragma unroll
+      for (int c = 0; c < N; c++) carry[c][g0 + c0 + i] = E0 + stage[c * kTupPad + at];
+    }
+    __syncthreads();  // the stage is rewritten by the next chunk
+  }
+}
+
+// The same for a group whose elements or tuples overflow 32 bits: slot by
+// slot (q = t, t + 128, ...), each by binary search and decode.
+template <int N>
+__device__ inline void group_tuples_wide(int64_t* const* carry, const int64_t* s_tb, const int64_t* s_off, int nl) {
+  const int64_t g0 = s_tb[0];
+  const int64_t tot = s_tb[nl] - g0;
+  for (int64_t q = threadIdx.x; q < tot; q += kPRLists) {
+    const int64_t slot = g0 + q;
+    int lo = 0, hi = nl;  // the last list e with s_tb[e] <= slot
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_tb[mid] <= slot) lo = mid;
+      else hi = mid;
+    }
+    int64_t x[N];
+    decode_tuple<N>(slot - s_tb[lo], s_off[lo + 1] - s_off[lo], x);
+#pragma unroll
+    for (int c = 0; c < N; c++) carry[c][slot] = s_off[lo] + x[c];
+  }
+}
+
 template <typename F>
 __device__ inline F as_value(uint64_t w) {
   if constexpr (sizeof(F) == 4) return __uint_as_float((uint32_t)w);
@@ -70,21 +203,50 @@ __device__ inline void eval_staged(const ExprProgram& p, const PairSlots& ps, co
 }
 
 template <typename W, int U, typename F>
-__global__ __launch_bounds__(kPRLists) void pair_reduce_kernel(ExprProgram p, PairSlots ps,
-                                                               const int64_t* __restrict__ offsets, int64_t n,
-                                                               PairOut out) {
-  extern __shared__ uint64_t s_dyn[];  // ps.nslots columns of kPRStage values
+__global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlots ps, int prog,
+                                                         const int64_t* __restrict__ offsets, int64_t n, PairOut out,
+                                                         CombReq cr) {
+  // dynamic LDS: the tuple stage (phase A), then the staged columns and the
+  // pair values (phase B)
+  extern __shared__ uint64_t s_dyn[];
+  int* stage = (int*)s_dyn;
   W* s_col = (W*)s_dyn;
+  F* s_val = (F*)(s_col + (ps.nslots > 0 ? ps.nslots : 1) * kPRStage);
   __shared__ int64_t s_off[kPRLists + 1];
+  __shared__ int64_t s_tb[kPRLists + 1];
+  __shared__ int s_offl[kPRLists + 1];
+  __shared__ int s_tbl[kPRLists + 1];
   __shared__ int s_pbeg[kPRLists + 1];
   __shared__ int s_wsum[kPRLists / kWave];
-  __shared__ F s_val[kPRCap];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int64_t L0 = (int64_t)blockIdx.x * kPRLists;
   const int nl = (int)((n - L0) < kPRLists ? (n - L0) : kPRLists);
   for (int t = tid; t <= nl; t += kPRLists) s_off[t] = offsets[L0 + t];
-  __syncthreads();
+  // phase A: the carries of the requested combinations (write-bound; the
+  // program's evaluation below is ALU-bound, and the workgroups on a CU
+  // overlap the two)
+  for (int k = 0; k < cr.nreq; k++) {
+    for (int t = tid; t <= nl; t += kPRLists) s_tb[t] = cr.tooffsets[k][L0 + t];
+    __syncthreads();
+    const int64_t E0 = s_off[0], g0 = s_tb[0];
+    const bool narrow = s_off[nl] - E0 < (int64_t)INT32_MAX && s_tb[nl] - g0 < (int64_t)INT32_MAX;  // uniform
+    if (narrow) {
+      for (int t = tid; t <= nl; t += kPRLists) {
+        s_offl[t] = (int)(s_off[t] - E0);
+        s_tbl[t] = (int)(s_tb[t] - g0);
+      }
+      __syncthreads();
+      if (cr.n[k] == 2) group_tuples<2>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage);
+      else group_tuples<3>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage);
+    } else {
+      if (cr.n[k] == 2) group_tuples_wide<2>(cr.carry[k], s_tb, s_off, nl);
+      else group_tuples_wide<3>(cr.carry[k], s_tb, s_off, nl);
+    }
+    __syncthreads();  // s_tb is rewritten by the next request
+  }
+  if (!prog) return;  // uniform
+  __syncthreads();    // s_off (with no requests, nothing above waited for it)
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
   const bool staged = span <= kPRStage;
@@ -221,15 +383,23 @@ __global__ __launch_bounds__(kPRLists) void pair_reduce_kernel(ExprProgram p, Pa
 }
 
 template <typename F>
-inline void launch_pairs(const ExprProgram& p, const PairSlots& ps, const int64_t* offsets, int64_t n,
-                         const PairOut& out) {
+inline void launch_pairs(const ExprProgram& p, const PairSlots& ps, int prog, const int64_t* offsets, int64_t n,
+                         const PairOut& out, const CombReq& cr) {
   const unsigned grid = (unsigned)ceil_div(n, (int64_t)kPRLists);
+  int maxn = 0;
+  for (int k = 0; k < cr.nreq; k++) maxn = cr.n[k] > maxn ? cr.n[k] : maxn;
+  const size_t lds_a = (size_t)maxn * kTupPad * sizeof(int);
+  if (!prog) {  // carries only: no staged columns
+    pairs_kernel<uint32_t, 4, F><<<grid, kPRLists, lds_a, stream()>>>(p, ps, 0, offsets, n, out, cr);
+    return;
+  }
   const size_t w = p.wide ? 8 : 4;
-  const size_t lds = (size_t)(ps.nslots > 0 ? ps.nslots : 1) * kPRStage * w;
+  size_t lds = (size_t)(ps.nslots > 0 ? ps.nslots : 1) * kPRStage * w + kPRCap * sizeof(F);
+  if (lds < lds_a) lds = lds_a;
   expr_dispatch(p, [&](auto kind) {
     using W = typename decltype(kind)::Word;
     constexpr int U = decltype(kind)::slots;
-    pair_reduce_kernel<W, U, F><<<grid, kPRLists, lds, stream()>>>(p, ps, offsets, n, out);
+    pairs_kernel<W, U, F><<<grid, kPRLists, lds, stream()>>>(p, ps, 1, offsets, n, out, cr);
   });
 }
 
@@ -259,6 +429,11 @@ using namespace akhip;
 
 extern "C" {
 
+Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t nreq, const int32_t* req_n,
+                                     const int64_t* const* req_tooffsets, int64_t* const* req_carries,
+                                     const ExprProgram* program, void* out_max, void* out_min, void* out_sum,
+                                     int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
+
 // Reductions over the pairs (i < j, lexicographic) of the n lists of int64
 // offsets (absolute positions in the program's columns): every output may be
 // null; max / min / sum have the program's out_type (float32 or float64),
@@ -268,16 +443,45 @@ extern "C" {
 Error awkward_hip_pair_reduce(const int64_t* offsets, int64_t n, const ExprProgram* program, void* out_max,
                               void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin,
                               int8_t* out_mask) {
-  const ExprProgram p = *program;
-  if (!expr_program_ok(p)) return failure("malformed pair program", kSliceNone, kSliceNone, kFile);
-  if (p.out_type != kTFloat32 && p.out_type != kTFloat64)
-    return failure("pair reductions take float32 or float64 programs", kSliceNone, kSliceNone, kFile);
-  if (n <= 0) return success();
+  return awkward_hip_combinations_pairs(offsets, n, 0, nullptr, nullptr, nullptr, program, out_max, out_min,
+                                        out_sum, out_argmax, out_argmin, out_mask);
+}
+
+// ak.combinations(array, k) for up to two k in {2, 3} (no replacement) and,
+// optionally, the pair reductions of awkward_hip_pair_reduce, in one launch.
+// req_n, req_tooffsets (the k-th request's tuple offsets: the output of
+// combinations_length for it) and req_carries (req_n[0] carry pointers,
+// then req_n[1]) are HOST arrays of device pointers; program may be NULL
+// (no reductions: the outputs are then ignored).
+Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t nreq, const int32_t* req_n,
+                                     const int64_t* const* req_tooffsets, int64_t* const* req_carries,
+                                     const ExprProgram* program, void* out_max, void* out_min, void* out_sum,
+                                     int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask) {
+  if (nreq < 0 || nreq > 2) return failure("at most two combinations per launch", kSliceNone, kSliceNone, kFile);
+  CombReq cr{};
+  cr.nreq = nreq;
+  int used = 0;
+  for (int k = 0; k < nreq; k++) {
+    if (req_n[k] != 2 && req_n[k] != 3)
+      return failure("fused combinations take n = 2 or 3", kSliceNone, kSliceNone, kFile);
+    cr.n[k] = req_n[k];
+    cr.tooffsets[k] = req_tooffsets[k];
+    for (int c = 0; c < req_n[k]; c++) cr.carry[k][c] = req_carries[used++];
+  }
+  ExprProgram p{};
+  const int prog = program != nullptr;
+  if (prog) {
+    p = *program;
+    if (!expr_program_ok(p)) return failure("malformed pair program", kSliceNone, kSliceNone, kFile);
+    if (p.out_type != kTFloat32 && p.out_type != kTFloat64)
+      return failure("pair reductions take float32 or float64 programs", kSliceNone, kSliceNone, kFile);
+  }
+  if (n <= 0 || (!prog && nreq == 0)) return success();
   const PairOut out{out_max, out_min, out_sum, out_argmax, out_argmin, out_mask};
-  const PairSlots ps = pair_slots(p);
-  if (p.out_type == kTFloat32) launch_pairs<float>(p, ps, offsets, n, out);
-  else launch_pairs<double>(p, ps, offsets, n, out);
-  AKHIP_LAUNCH_CHECK("pair_reduce_kernel");
+  const PairSlots ps = prog ? pair_slots(p) : PairSlots{};
+  if (prog && p.out_type == kTFloat64) launch_pairs<double>(p, ps, prog, offsets, n, out, cr);
+  else launch_pairs<float>(p, ps, prog, offsets, n, out, cr);
+  AKHIP_LAUNCH_CHECK("pairs_kernel");
   return success();
 }
 

@@ -186,6 +186,79 @@ class PairReduce:
         return o
 
 
+class _Tuples:
+    """One n's outputs of a CombinationsPairs launch (the shape of
+    Combinations's: tooffsets, total, carry)."""
+
+    def __init__(self, tooffsets, totallen):
+        self.tooffsets, self.totallen = tooffsets, totallen
+        self.total, self.carry = 0, []
+
+
+class CombinationsPairs:
+    """ak.combinations(ev, n) for up to two n in {2, 3} (no replacement) and,
+    optionally, the pair reductions of one expression (as PairReduce), in one
+    launch (``awkward_hip_combinations_pairs``) after one
+    combinations_length scan per n: each workgroup writes its lists' carries
+    (write-bound) and evaluates and reduces the program over their pairs
+    (ALU-bound), so the two overlap on the chip.  The tuple counts are read
+    once at construction to size the carries; run() is capturable."""
+
+    def __init__(self, events, ns, expr=None, outputs=("max", "argmax")):
+        layout = events.layout
+        self.offsets = layout.offsets
+        self.length = layout.length
+        self.ns = tuple(ns)
+        if not 1 <= len(self.ns) <= 2 or any(n not in (2, 3) for n in self.ns) or len(set(self.ns)) != len(self.ns):
+            raise ValueError("one or two distinct tuple sizes from {2, 3}")
+        dev = self.offsets.device
+        self.starts, self.stops = self.offsets[:-1], self.offsets[1:]
+        self.tuples = {n: _Tuples(torch.empty(self.length + 1, dtype=torch.int64, device=dev),
+                                  torch.empty(1, dtype=torch.int64, device=dev)) for n in self.ns}
+        P, I64, I32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        self._len = _fn("awkward_ListArray64_combinations_length_64", [P, P, I64, ctypes.c_bool, P, P, I64])
+        self._f = _fn("awkward_hip_combinations_pairs", [P, I64, I32, P, P, P, P, P, P, P, P, P, P])
+        self._run_lengths()
+        torch.cuda.synchronize()
+        carries = []
+        for n in self.ns:
+            t = self.tuples[n]
+            t.total = int(t.totallen.item())
+            t.carry = [torch.empty(max(t.total, 1), dtype=torch.int64, device=dev) for _ in range(n)]
+            carries.extend(t.carry)
+        k = len(self.ns)
+        self.req_n = (ctypes.c_int32 * k)(*self.ns)
+        self.req_tooffsets = (ctypes.c_void_p * k)(*[self.tuples[n].tooffsets.data_ptr() for n in self.ns])
+        self.req_carries = (ctypes.c_void_p * len(carries))(*[c.data_ptr() for c in carries])
+        self.out = {}
+        self.program = None
+        if expr is not None:
+            pr = PairReduce(self.offsets, expr, outputs)
+            self.program, self._keep, self.out = pr.program, pr._keep, pr.out
+
+    def _run_lengths(self):
+        set_stream()
+        for n in self.ns:
+            t = self.tuples[n]
+            _check(self._len(t.totallen.data_ptr(), t.tooffsets.data_ptr(), n, False, self.starts.data_ptr(),
+                             self.stops.data_ptr(), self.length))
+
+    def run(self):
+        self._run_lengths()
+        o = self.out
+        _check(self._f(device_address(self.offsets), self.length, len(self.ns), ctypes.addressof(self.req_n),
+                       ctypes.addressof(self.req_tooffsets), ctypes.addressof(self.req_carries),
+                       ctypes.addressof(self.program) if self.program is not None else 0,
+                       *[o[k].data_ptr() if k in o else 0 for k in PairReduce._SLOTS]))
+        return self
+
+    def algorithmic_bytes(self):
+        """Offsets read once (and once per scan), tooffsets written, every
+        carry written once."""
+        b = 8 * (self.length + 1) * (1 + 2 * len(self.ns))
+        return b + sum(8 * n * self.tuples[n].total for n in self.ns)
+
+
 class Combinations:
     """ak.combinations(ev, n)'s carries without a host round trip per call:
     the tuple count is read once at construction to size the outputs, then

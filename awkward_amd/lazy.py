@@ -521,6 +521,9 @@ class Plan:
         self.combos = {}  # combinations id -> node
         self._pairs = {}
         self._combos = {}
+        self.units = {}  # input id -> {"source", "ns": {n: [combinations keys]}, "pairs": key or None}
+        self._unit_of = {}  # combinations / pair key -> input id
+        self._runners = {}
         if not fuse:
             return
         f32 = np.dtype(np.float32)
@@ -540,6 +543,7 @@ class Plan:
             if dtypes.get(field) != f32:  # the kernel reduces float32 values
                 continue
             self._add(out, source, terms, sl, field, name)
+        self._plan_units(fuse)
         for out, source, terms, sl in pending_num:
             base = (source.id, terms, (sl.start, sl.stop, sl.step))
             key = next((k for k in self.groups if k[:3] == base), None)
@@ -568,9 +572,45 @@ class Plan:
             self.combos[key] = node
             self.fused_nodes[node.id] = key
 
+    def _plan_units(self, fuse):
+        """COMBINATIONS outputs with n in {2, 3} (no replacement) on one input,
+        and one pair reduction group over that input's pairs, become one
+        awkward_hip_combinations_pairs launch."""
+        if not fuse:
+            return
+        for key, node in self.combos.items():
+            n = node.params["n"]
+            if n not in (2, 3) or node.params["replacement"]:
+                continue
+            src = node.inputs[0]
+            u = self.units.setdefault(src.id, {"source": src, "ns": {}, "pairs": None})
+            u["ns"].setdefault(n, []).append(key)
+            self._unit_of[key] = src.id
+        for key, g in self.pair_groups.items():
+            u = self.units.get(g["comb"].inputs[0].id)
+            if u is not None and u["pairs"] is None:
+                u["pairs"] = key
+                self._unit_of[key] = g["comb"].inputs[0].id
+
+    def _unit(self, uid):
+        from awkward_amd.fused import CombinationsPairs
+
+        r = self._runners.get(uid)
+        if r is None:
+            u = self.units[uid]
+            expr, outputs = None, ()
+            if u["pairs"] is not None:
+                g = self.pair_groups[u["pairs"]]
+                expr, outputs = g["expr"], tuple(o for o in _PAIR_OPS if o in g["ops"])
+            r = CombinationsPairs(u["source"].params["array"], sorted(u["ns"]), expr, outputs)
+            self._runners[uid] = r
+        return r
+
     def _pair_reduce(self, key):
         from awkward_amd.fused import PairReduce
 
+        if key in self._unit_of:
+            return self._unit(self._unit_of[key])
         p = self._pairs.get(key)
         if p is None:
             g = self.pair_groups[key]
@@ -582,6 +622,8 @@ class Plan:
     def _combinations(self, key):
         from awkward_amd.fused import Combinations as Decode
 
+        if key in self._unit_of:
+            return self._unit(self._unit_of[key]).tuples[self.combos[key].params["n"]]
         d = self._combos.get(key)
         if d is None:
             node = self.combos[key]
@@ -636,14 +678,20 @@ class Plan:
                 "max": lambda o=o, be=be: Array(C.ByteMaskedArray(o["max_mask"], C.NumpyArray(o["max"], be), False,
                                                                   be)),
             }
+        for uid in self.units:
+            self._unit(uid).run()
         for key in self.pair_groups:
-            o = self._pair_reduce(key).run()
+            r = self._pair_reduce(key)
+            if key not in self._unit_of:
+                r.run()
+            o = r.out
             be = _list_source(self.pair_groups[key]["comb"].inputs[0]).backend
             masked = {k: lambda o=o, be=be, k=k: Array(C.ByteMaskedArray(o["mask"], C.NumpyArray(o[k], be), False, be))
                       for k in ("max", "min", "argmax", "argmin")}
             results[key] = dict(masked, sum=lambda o=o, be=be: Array(C.NumpyArray(o["sum"], be)))
         for key in self.combos:
-            self._combinations(key).run()
+            if key not in self._unit_of:
+                self._combinations(key).run()
         out = []
         for lz in self.outputs:
             key = self.fused_nodes.get(lz.node.id)
@@ -668,8 +716,10 @@ class Plan:
 
         if not self.fused_nodes or any(lz.node.id not in self.fused_nodes for lz in self.outputs):
             raise ValueError("only plans whose every output is fused can be captured")
-        reductions = [self._pipeline(key) for key in self.groups] + [self._pair_reduce(k) for k in self.pair_groups]
-        decodes = [self._combinations(key) for key in self.combos]
+        reductions = [self._pipeline(key) for key in self.groups] + \
+            [self._pair_reduce(k) for k in self.pair_groups if k not in self._unit_of] + \
+            [self._unit(uid) for uid in self.units]
+        decodes = [self._combinations(key) for key in self.combos if key not in self._unit_of]
         if reductions and decodes:
             # the reductions (latency-bound) and the carries (write-bound)
             # are independent: two branches of the graph
@@ -692,8 +742,9 @@ class Plan:
         for lz in self.outputs:
             walk(lz.node)
         fused = len(self.fused_nodes)
-        return {"nodes": len(seen), "fused_reductions": fused - len(self.combos),
-                "fused_launches": len(self.groups) + len(self.pair_groups) + len(self.combos),
+        in_units = sum(1 for k in list(self.pair_groups) + list(self.combos) if k in self._unit_of)
+        launches = len(self.groups) + len(self.pair_groups) + len(self.combos) - in_units + len(self.units)
+        return {"nodes": len(seen), "fused_reductions": fused - len(self.combos), "fused_launches": launches,
                 "eager_outputs": len(self.outputs) - fused}
 
 

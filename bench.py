@@ -173,6 +173,15 @@ def pairs_plan(ev):
     return Plan([expr.max(), expr.argmax(), pairs, lz.combinations(3)])
 
 
+def pairs_expr_device(ev):
+    """The pair expression as a leg-aware device expression over ev's columns
+    (what the lazy plan compiles for pairs_plan)."""
+    from awkward_amd import lazy as L
+
+    plan = pairs_plan(ev)
+    return next(iter(plan.pair_groups.values()))["expr"]
+
+
 def same_outputs(torch, a, b):
     """Bit equality of pairs_step's four outputs (masked max / argmax, two
     combinations layouts) between two forms."""
@@ -197,6 +206,8 @@ def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
     the n=3 combinations)."""
     import numpy as np
 
+    from awkward_amd.fused import Combinations, PairReduce
+
     off, fields = make_events(n_events, seed=3, lam=4, clip=30)
     fields = {k: fields[k] for k in ("pt", "eta", "phi")}
     ev = ak.from_offsets(off, fields, backend)
@@ -205,21 +216,23 @@ def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
     out["plain"] = time_form(torch, lambda: pairs_step(ak, np, ev), steps, 1, n_events)
     plan = pairs_plan(ev)
     stats = plan.stats()
-    assert stats["eager_outputs"] == 0 and stats["fused_launches"] == 3, stats
+    assert stats["eager_outputs"] == 0 and stats["fused_launches"] == 1, stats
     fused_out = plan.run()
     plain_out = pairs_step(ak, np, ev)
     out["outputs_equal"] = same_outputs(torch, fused_out, plain_out)
-    out["pairs"] = int(plain_out[2].layout.offsets[-1].item())
-    del plain_out
-    pr = plan._pair_reduce(next(iter(plan.pair_groups)))
-    tri = plan._combinations(next(k for k, n in plan.combos.items() if n.params["n"] == 3))
-    out["triples"] = tri.total
+    del plain_out, fused_out
+    unit = plan._unit(next(iter(plan.units)))
+    out["pairs"], out["triples"] = unit.tuples[2].total, unit.tuples[3].total
     graph = plan.graph()
     out["fused_graph"] = time_form(torch, graph.replay, steps, 1, n_events)
     out["speedup"] = round(out["fused_graph"]["events_per_s"] / out["plain"]["events_per_s"], 2)
-    # the parts alone: the pair kernel (latency-bound) and the triples'
-    # decode (write-bound, 8 * 3 * P bytes)
-    for name, fn, nbytes in (("pair_reduce", pr.run, None), ("combinations3", tri.run, tri.algorithmic_bytes())):
+    # the parts: the fused launch with its two length scans (writes the 2.56 GB
+    # of pairs and 5.12 GB of triples while reducing the expression), the pair
+    # reductions alone (ALU-bound) and the triples' decode alone (write-bound)
+    pr = PairReduce(ev.layout.offsets, pairs_expr_device(ev), ("max", "argmax"))
+    tri = Combinations(ev, 3)
+    for name, fn, nbytes in (("combinations_pairs", unit.run, unit.algorithmic_bytes()), ("pair_reduce", pr.run, None),
+                             ("combinations3", tri.run, tri.algorithmic_bytes())):
         fn()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
@@ -232,8 +245,9 @@ def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
             out[name].update({"GB/s": round(nbytes / ms / 1e6, 1), "bytes": nbytes})
     out["what"] = ("plain: ak.combinations(ev,2) + the pair expression as one expression-VM pass over the carried "
                    "legs + ak.max/argmax + ak.combinations(ev,3); fused_graph: the same graph recorded with "
-                   "lazy(ev) -- awkward_hip_pair_reduce over the pairs in place, beside the n=2 and n=3 carries' "
-                   "decodes, as two concurrent branches of one hipGraph")
+                   "lazy(ev) -- two combinations_length scans and one awkward_hip_combinations_pairs launch that "
+                   "writes the n=2 and n=3 carries and reduces the expression over the pairs in place, replayed "
+                   "from a hipGraph")
     return out
 
 

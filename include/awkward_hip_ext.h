@@ -276,6 +276,15 @@ struct Error awkward_hip_pair_expr(float* out, const float* pt1, const float* et
      program's type, mask is 1 for a list without pairs; any output may be
      NULL. */
 struct Error awkward_hip_pair_reduce(const int64_t* offsets, int64_t n, const void* program, void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
+/* combinations_pairs: ak.combinations(array, k) for up to two k in {2, 3}
+     (without replacement) -- the reference's awkward_ListArray_combinations
+     per k -- and, when program is not NULL, pair_reduce's reductions, in one
+     launch.  req_n (nreq sizes), req_tooffsets (for each request the n + 1
+     tuple offsets that awkward_ListArray64_combinations_length_64 wrote for
+     these offsets) and req_carries (req_n[0] carry pointers, then req_n[1])
+     are HOST arrays of device pointers; every carry holds the request's
+     tuple count; carries are absolute element positions. */
+struct Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t nreq, const int32_t* req_n, const int64_t* const* req_tooffsets, int64_t* const* req_carries, const void* program, void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
 
 /* ---- batch runner: many arrays of one dtype, persistent work queues ------------
    items: device table of 64-byte rows {data, offsets, out_sum, out_min,

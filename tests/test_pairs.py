@@ -1,8 +1,9 @@
 """Reductions over the pairs of ak.combinations(ev, 2), recorded lazily and
-fused into one pair-kernel launch (awkward_hip_pair_reduce) -- checked bit
-for bit against the same graph evaluated node by node through the host
-layer, and against the oracle where the expression's ops are exact IEEE
-arithmetic."""
+fused into one pair-kernel launch (awkward_hip_pair_reduce), and with the
+plan's n = 2 / n = 3 combinations outputs into one
+awkward_hip_combinations_pairs launch -- checked bit for bit against the same
+graph evaluated node by node through the host layer, and against the oracle
+where the expression's ops are exact IEEE arithmetic."""
 
 import numpy as np
 import pytest
@@ -76,7 +77,7 @@ def test_fused_pair_reductions_equal_eager(make):
     outs = reductions(mass_like(pairs)) + [pairs, ev.combinations(3)]
     plan = L.Plan(outs)
     stats = plan.stats()
-    assert stats["fused_reductions"] == 5 and stats["fused_launches"] == 3 and stats["eager_outputs"] == 0
+    assert stats["fused_reductions"] == 5 and stats["fused_launches"] == 1 and stats["eager_outputs"] == 0
     fused = plan.run()
     eager = L.compute(*outs, fuse=False)
     for f, e in zip(fused[:4], eager[:4]):
@@ -157,15 +158,26 @@ def test_pair_plan_graph_replays():
     ev = L.lazy(ak.from_offsets(off, fields, HipBackend()))
     pairs = ev.combinations(2)
     expr = mass_like(pairs)
-    plan = L.Plan([expr.max(), expr.argmax(), pairs, ev.combinations(3)])
-    want = L.compute(expr.max(), expr.argmax(), fuse=False)
+    triples = ev.combinations(3)
+    plan = L.Plan([expr.max(), expr.argmax(), pairs, triples])
+    want = L.compute(expr.max(), expr.argmax(), pairs, triples, fuse=False)
     graph = plan.graph()
     first = None
     for _ in range(3):
         graph.replay()
         torch.cuda.synchronize()
-        got = plan._pair_reduce(next(iter(plan.pair_groups))).out
+        unit = plan._unit(next(iter(plan.units)))
+        got = dict(unit.out)
+        for n in (2, 3):
+            for c, carry in enumerate(unit.tuples[n].carry):
+                got[f"carry{n}{c}"] = carry[:unit.tuples[n].total]
+            got[f"offsets{n}"] = unit.tuples[n].tooffsets
         snap = {k: v.clone() for k, v in got.items()}
+        if first is None:
+            for n, w in ((2, want[2]), (3, want[3])):
+                assert torch.equal(snap[f"offsets{n}"], w.layout.offsets)
+                for c, content in enumerate(w.layout.content.contents):
+                    assert torch.equal(snap[f"carry{n}{c}"], content.index)
         if first is None:
             first = snap
             mx = want[0].layout
@@ -173,3 +185,21 @@ def test_pair_plan_graph_replays():
         else:
             assert all(torch.equal(first[k], snap[k]) for k in first)
         _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+
+
+@pytest.mark.gpu
+def test_combinations_outputs_alone_and_with_replacement():
+    """Carries only (no reductions) in one launch for n = 2 and 3; n = 3 with
+    replacement and n = 4 keep their own decodes."""
+    from awkward_amd.backend import HipBackend
+
+    off, fields = heavy_events(6000, seed=14)
+    ev = L.lazy(ak.from_offsets(off, fields, HipBackend()))
+    outs = [ev.combinations(2), ev.combinations(3), ev.combinations(3, replacement=True), ev.combinations(4)]
+    plan = L.Plan(outs)
+    assert len(plan.units) == 1 and plan.stats()["fused_launches"] == 3
+    for f, e in zip(plan.run(), L.compute(*outs, fuse=False)):
+        lf, le = f.layout, e.layout
+        assert np.array_equal(lf.nplike.to_numpy(lf.offsets), le.nplike.to_numpy(le.offsets))
+        for cf, ce in zip(lf.content.contents, le.content.contents):
+            assert np.array_equal(cf.nplike.to_numpy(cf.index), ce.nplike.to_numpy(ce.index))
