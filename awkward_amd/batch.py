@@ -13,6 +13,10 @@ The per-array results match the registry path on the same array:
 ``ak.sum``, ``ak.min`` / ``ak.argmin`` (mask_identity=True: value + empty
 mask) and ``ak.argsort`` (stable, ascending).
 
+``WaveRunner`` streams a batch larger than HBM through ``BatchRunner`` in
+waves: consecutive arrays whose device bytes (``device_bytes``) fit a budget
+are made on the device, run and dropped before the next wave.
+
 ``variants`` selects the reduce tile per dtype (the ten tiles of
 ``dispatch.VARIANTS``); the autotuner (``tune()``) times them on the batch
 itself and keeps the fastest.  ``variants="learned"`` instead enqueues, per
@@ -248,3 +252,69 @@ class BatchRunner:
             g._build(self.arrays, self.arrays[0][0].device)
             timings[g.name] = {"best": best, "ms": times}
         return timings
+
+
+def release_pools():
+    """Frees the current stream's scratch and queue pools (after synchronising
+    it; the next call allocates afresh): call only when no captured hipGraph
+    still uses them."""
+    f = lib().awkward_hip_release_pools
+    f.restype = ctypes.c_int
+    set_stream()
+    if f() != 0:
+        raise RuntimeError("awkward_hip_release_pools failed")
+
+
+def device_bytes(n_lists: int, n_elements: int, itemsize: int, argsort: bool = True) -> int:
+    """HBM that one array holds while a BatchRunner runs it: offsets,
+    content, the four reduce outputs (the sum counted at 8 bytes, its widest
+    type), and (argsort) its int64 output plus its share of the sort's
+    scratch (24 bytes per element of the dtype group)."""
+    b = 8 * (n_lists + 1) + itemsize * n_elements + n_lists * (8 + itemsize + 8 + 1)
+    if argsort:
+        b += 8 * n_elements + 24 * n_elements
+    return b
+
+
+def plan_waves(sizes, budget: int):
+    """Consecutive runs of array indices whose bytes stay within `budget`
+    (an array larger than the budget is a wave of its own)."""
+    waves, cur, used = [], [], 0
+    for i, b in enumerate(sizes):
+        if cur and used + b > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += b
+    if cur:
+        waves.append(cur)
+    return waves
+
+
+class WaveRunner:
+    """A batch streamed through the device in waves.
+
+    ``specs``: per array (n_lists, n_elements, numpy dtype); ``make(i)``
+    returns array i's (offsets, content) device tensors (copied or generated
+    on demand).  Iterating yields ``(indices, arrays)`` per wave -- run them
+    (a BatchRunner over ``arrays``, or the registry path) and read the
+    results inside the loop body: the wave's tensors and the library's
+    scratch pools are released before the next wave is made (so hipGraphs
+    captured over a wave must not outlive it)."""
+
+    def __init__(self, specs, make, budget_bytes: int, argsort: bool = True):
+        self.specs = list(specs)
+        self.make = make
+        self.sizes = [device_bytes(n, N, np.dtype(dt).itemsize, argsort) for n, N, dt in self.specs]
+        self.waves = plan_waves(self.sizes, budget_bytes)
+
+    def __iter__(self):
+        for wave in self.waves:
+            arrays = [self.make(i) for i in wave]
+            yield wave, arrays
+            del arrays
+            release_pools()
+            torch.cuda.empty_cache()
+
+    def bytes(self, wave):
+        return sum(self.sizes[i] for i in wave)

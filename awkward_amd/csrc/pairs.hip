@@ -147,10 +147,28 @@ __device__ inline void group_tuples(int64_t* const* carry, int64_t g0, int64_t E
       }
     }
     __syncthreads();
-    for (int i = tid; i < cn; i += kPRLists) {
-      const int at = tup_pad(i);
+    // two consecutive slots per lane, one 16-byte store per carry: the
+    // pairs start at an even absolute slot (an odd first slot goes alone)
+    const int64_t gbase = g0 + c0;
+    const int lead = (int)(gbase & 1);
+    if (lead && tid == 0) {
 #pragma unroll
-      for (int c = 0; c < N; c++) carry[c][g0 + c0 + i] = E0 + stage[c * kTupPad + at];
+      for (int c = 0; c < N; c++) carry[c][gbase] = E0 + stage[c * kTupPad + tup_pad(0)];
+    }
+    for (int i = lead + 2 * tid; i < cn; i += 2 * kPRLists) {
+      const int a0 = tup_pad(i), a1 = tup_pad(i + 1);
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        int64_t* dst = carry[c] + gbase;
+        const int64_t v0 = E0 + stage[c * kTupPad + a0];
+        if (i + 1 < cn) {
+          typedef long long v2 __attribute__((ext_vector_type(2)));
+          const v2 v = {(long long)v0, (long long)(E0 + stage[c * kTupPad + a1])};
+          *(v2*)(dst + i) = v;
+        } else {
+          dst[i] = v0;
+        }
+      }
     }
     __syncthreads();  // the stage is rewritten by the next chunk
   }
@@ -202,7 +220,7 @@ __device__ inline void eval_staged(const ExprProgram& p, const PairSlots& ps, co
   expr_run<U, W>(p, f, out);
 }
 
-template <typename W, int U, typename F>
+template <typename W, int U, typename F, bool PROG>
 __global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlots ps, int prog,
                                                          const int64_t* __restrict__ offsets, int64_t n, PairOut out,
                                                          CombReq cr) {
@@ -245,6 +263,7 @@ __global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlot
     }
     __syncthreads();  // s_tb is rewritten by the next request
   }
+  if constexpr (!PROG) return;
   if (!prog) return;  // uniform
   __syncthreads();    // s_off (with no requests, nothing above waited for it)
   const int64_t E0 = s_off[0];
@@ -390,7 +409,7 @@ inline void launch_pairs(const ExprProgram& p, const PairSlots& ps, int prog, co
   for (int k = 0; k < cr.nreq; k++) maxn = cr.n[k] > maxn ? cr.n[k] : maxn;
   const size_t lds_a = (size_t)maxn * kTupPad * sizeof(int);
   if (!prog) {  // carries only: no staged columns
-    pairs_kernel<uint32_t, 4, F><<<grid, kPRLists, lds_a, stream()>>>(p, ps, 0, offsets, n, out, cr);
+    pairs_kernel<uint32_t, 4, F, false><<<grid, kPRLists, lds_a, stream()>>>(p, ps, 0, offsets, n, out, cr);
     return;
   }
   const size_t w = p.wide ? 8 : 4;
@@ -399,7 +418,7 @@ inline void launch_pairs(const ExprProgram& p, const PairSlots& ps, int prog, co
   expr_dispatch(p, [&](auto kind) {
     using W = typename decltype(kind)::Word;
     constexpr int U = decltype(kind)::slots;
-    pairs_kernel<W, U, F><<<grid, kPRLists, lds, stream()>>>(p, ps, 1, offsets, n, out, cr);
+    pairs_kernel<W, U, F, true><<<grid, kPRLists, lds, stream()>>>(p, ps, 1, offsets, n, out, cr);
   });
 }
 

@@ -18,9 +18,11 @@
 //     across the kernels of one entry point).  A pool that grows retires its
 //     old buffer instead of freeing it: kernels in flight and captured
 //     hipGraphs may still use it, so a retired buffer stays allocated until
-//     the process ends (pools double, so the retired buffers of a pool add
-//     up to less than its current size).  Nothing on the call path
-//     synchronises the device or frees memory.
+//     the process ends, or until the caller, knowing no graph uses them any
+//     more, releases the pools (awkward_hip_release_pools).  Pools double up to
+//     1 GiB, so the retired buffers of a pool add up to less than its
+//     current size; beyond, they grow to the request.  Nothing on the call
+//     path synchronises the device or frees memory.
 //
 // Scratch is stream-ordered: the kernels of one stream use its pools one
 // after another, so two streams never share scratch.  A hipGraph captured on
@@ -133,12 +135,15 @@ Error describe(const StreamState* s, uint64_t key) {
 }
 
 // Grow-only device buffer: at least `bytes`, contents not preserved on
-// growth; the old buffer is retired, not freed.
+// growth; the old buffer is retired, not freed.  Up to 1 GiB the size
+// doubles; beyond, it is the request rounded up to 64 MiB (doubling a
+// 100 GB request would hold 128 GiB, plus the retired buffers).
 void* grow(Pool* pool, size_t bytes) {
   if (bytes < 256) bytes = 256;
   if (pool->bytes < bytes) {
     size_t want = pool->bytes ? pool->bytes : (size_t)1 << 20;
-    while (want < bytes) want *= 2;
+    while (want < bytes && want < ((size_t)1 << 30)) want *= 2;
+    if (want < bytes) want = (bytes + ((size_t)64 << 20) - 1) / ((size_t)64 << 20) * ((size_t)64 << 20);
     void* p = nullptr;
     if (hipMalloc(&p, want) != hipSuccess) return nullptr;
     if (pool->buf) pool->retired.push_back(pool->buf);
@@ -255,6 +260,28 @@ int awkward_hip_reserve_scratch(int64_t bytes) {
 // The same for the work-queue pool (the batch reductions' long-list queue).
 int awkward_hip_reserve_queue(int64_t bytes) {
   return queue_pool((size_t)bytes) != nullptr ? 0 : 1;
+}
+
+// Frees the current stream's pool buffers, current and retired, after
+// synchronising the stream; the next use allocates afresh.  Only for a
+// caller that knows no captured hipGraph still uses them (e.g. between the
+// waves of a batch larger than HBM, once the wave's graphs are gone).
+// Returns 0 on success.
+int awkward_hip_release_pools(void) {
+  StreamState* s = state();
+  if (s == nullptr) return 1;
+  if (hipStreamSynchronize(t_stream) != hipSuccess) return 1;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  int rc = 0;
+  for (Pool* pool : {&s->scratch, &s->queue}) {
+    for (void* b : pool->retired)
+      if (hipFree(b) != hipSuccess) rc = 1;
+    pool->retired.clear();
+    if (pool->buf != nullptr && hipFree(pool->buf) != hipSuccess) rc = 1;
+    pool->buf = nullptr;
+    pool->bytes = 0;
+  }
+  return rc;
 }
 
 // Bytes held by the current stream's pools: current buffers and retired

@@ -70,13 +70,10 @@ def run_segmented_reductions(ak, backend, torch, reps=5):
     """Segmented reductions on 1M Poisson(10) float64 lists, HBM-cold."""
     import numpy as np
 
-    rng = np.random.default_rng(1)
-    counts = rng.poisson(10, 1_000_000)
-    off = np.zeros(len(counts) + 1, dtype=np.int64)
-    np.cumsum(counts, out=off[1:])
+    off, x = make_config1()
+    HOST["config1"] = (off, x)
     N = int(off[-1])
-    x = rng.standard_normal(N)
-    n = len(counts)
+    n = len(off) - 1
     dev_off = torch.from_numpy(off).cuda()
     dev_x = torch.from_numpy(x).cuda()
     scrub = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB > 256 MiB MALL
@@ -210,6 +207,7 @@ def run_combinations(ak, backend, torch, n_events=20_000_000, steps=5):
 
     off, fields = make_events(n_events, seed=3, lam=4, clip=30)
     fields = {k: fields[k] for k in ("pt", "eta", "phi")}
+    HOST["config3"] = (off, fields)
     ev = ak.from_offsets(off, fields, backend)
     torch.cuda.synchronize()
     out = {"events": n_events, "particles": int(off[-1])}
@@ -280,50 +278,176 @@ def make_batch(seed=4, narrays=256, min_exp=3.0, max_exp=5.0, zipf_cap=100_000):
     return out
 
 
-def run_batched(ak, torch, steps=3):
-    """Batched uneven workload: per array sum, min, argmin and argsort.
-    plain = the registry path array by array; batched = the persistent
-    work-queue kernels (autotuned tiles) replayed from one hipGraph."""
+def batch_specs(seed=4, narrays=256, min_exp=3.0, max_exp=7.0):
+    """The batched workload's arrays (config 4): per array a list count
+    10**U(min_exp, max_exp), list lengths Poisson(2) / Poisson(50) /
+    Zipf(1.5) capped at 1e5, 5% of the arrays all-empty, dtype int32 /
+    int64 / float32 / float64, values integers(-1000, 1000) or N(0, 1).  The
+    per-array parameters come from numpy's generator at `seed`; each array's
+    lengths and values are drawn on the device by torch's generator at its
+    own seed (make_batch_array)."""
     import numpy as np
 
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(narrays):
+        n = int(10 ** rng.uniform(min_exp, max_exp))
+        kind = int(rng.integers(0, 3))
+        empty = bool(rng.random() < 0.05)
+        dt = [np.int32, np.int64, np.float32, np.float64][int(rng.integers(0, 4))]
+        specs.append({"n": n, "kind": kind, "empty": empty, "dtype": np.dtype(dt), "seed": 1000 * seed + i})
+    return specs
+
+
+_ZIPF_CDF = {}
+
+
+def make_batch_offsets(torch, spec, cap=100_000):
+    """The offsets of one array of batch_specs, on the device (its lengths
+    drawn by torch's generator at the array's seed)."""
+    import numpy as np
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(spec["seed"])
+    n = spec["n"]
+    if spec["empty"]:
+        counts = torch.zeros(n, dtype=torch.int64, device="cuda")
+    elif spec["kind"] < 2:
+        lam = torch.full((n,), 2.0 if spec["kind"] == 0 else 50.0, dtype=torch.float32, device="cuda")
+        counts = torch.poisson(lam, generator=g).to(torch.int64)
+    else:
+        cdf = _ZIPF_CDF.get(cap)
+        if cdf is None:  # P(X = k) = k^-1.5 / zeta(1.5) for k < cap; the rest of the mass at cap
+            from scipy.special import zeta
+
+            k = np.arange(1, cap, dtype=np.float64)
+            cdf = torch.from_numpy(np.cumsum(k ** -1.5) / zeta(1.5)).cuda()
+            _ZIPF_CDF[cap] = cdf
+        u = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+        counts = (torch.searchsorted(cdf, u) + 1).clamp_(max=cap)
+    off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    torch.cumsum(counts, 0, out=off[1:])
+    return off
+
+
+def make_batch_array(torch, spec):
+    """One array of batch_specs on the device: (offsets, content); the values
+    come from a second generator, seeded from the array's seed."""
+    import numpy as np
+
+    off = make_batch_offsets(torch, spec)
+    N = int(off[-1].item())
+    g = torch.Generator(device="cuda")
+    g.manual_seed(spec["seed"] + 500_000)
+    tdt = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64, np.dtype(np.float32): torch.float32,
+           np.dtype(np.float64): torch.float64}[spec["dtype"]]
+    if spec["dtype"].kind == "i":
+        x = torch.randint(-1000, 1000, (N,), dtype=tdt, device="cuda", generator=g)
+    else:
+        x = torch.randn(N, dtype=tdt, device="cuda", generator=g)
+    return off, x
+
+
+def run_batched(ak, torch, budget_gb=96.0, cpu_elements=20_000_000):
+    """Config 4 at its stated size: 256 arrays, list counts log-uniform in
+    [1e3, 1e7], per array sum, min, argmin and argsort.  The arrays are made
+    on the device and streamed in waves of at most `budget_gb` (WaveRunner);
+    per wave, each form runs once untimed and once timed: plain = the
+    registry path array by array; batched = the persistent work-queue
+    kernels (each dtype's tile autotuned on its first wave), one launch per dtype group
+    and op, enqueued directly (a hipGraph would warm up and capture on a
+    stream of its own, whose scratch pool would double the wave's HBM);
+    batched_learned = the same with the tile chosen on the device by the
+    learned dispatcher.  Times and bytes are summed over the waves."""
+    import numpy as np
+
+    from awkward_amd import contents as C
     from awkward_amd.backend import HipBackend
-    from awkward_amd.batch import BatchRunner
-    from awkward_amd.fused import Graph
+    from awkward_amd.batch import BatchRunner, WaveRunner
+    from awkward_amd.highlevel import Array
 
-    batch = make_batch()
-    n_lists = int(sum(len(o) - 1 for o, _ in batch))
-    n_elem = int(sum(len(x) for _, x in batch))
+    specs = batch_specs()
     be = HipBackend(fused=False)
-    arrays = [ak.from_offsets(o, x, be) for o, x in batch]
-    torch.cuda.synchronize()
+    # the list and element counts, before any wave (lengths drawn on the device)
+    sizes = []
+    for sp in specs:
+        off = make_batch_offsets(torch, sp)
+        sizes.append((sp["n"], int(off[-1].item()), sp["dtype"]))
+        del off
+    torch.cuda.empty_cache()
+    waves = WaveRunner(sizes, lambda i: make_batch_array(torch, specs[i]), int(budget_gb * 2**30))
+    total = {"plain": 0.0, "batched": 0.0, "batched_learned": 0.0}
+    realized = 0
+    tiles = {}
+    learned_tiles = {}
+    host_prefix, prefix_elems, float_samples = [], 0, []
 
-    def plain():
-        for a in arrays:
-            ak.sum(a, axis=-1)
-            ak.min(a, axis=-1)
-            ak.argmin(a, axis=-1)
-            ak.argsort(a, axis=-1)
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
 
-    out = {"arrays": len(batch), "lists": n_lists, "elements": n_elem,
-           "sample": "list counts log-uniform in [1e3, 1e5] (the configuration's [1e3, 1e7] would hold ~2.6e10 "
-                     "elements); every other parameter as configured, seed 4"}
-    out["plain"] = time_form(torch, plain, steps, 1, n_lists)
-    dev = [(a.layout.offsets, a.layout.content.data) for a in arrays]
-    runner = BatchRunner(dev)
-    out["autotune"] = runner.tune(reps=3)
-    graph = Graph(runner.run)
-    out["batched_graph"] = time_form(torch, graph.replay, steps * 3, 1, n_lists)
-    # the same batch with the tile chosen on the device by the learned dispatcher
-    learned_runner = BatchRunner(dev, variants="learned")
-    lgraph = Graph(learned_runner.run)
-    out["batched_learned_graph"] = time_form(torch, lgraph.replay, steps * 3, 1, n_lists)
-    out["learned_tiles"] = {g.name: g.chosen() for g in learned_runner.groups}
-    for form in ("plain", "batched_graph", "batched_learned_graph"):
-        out[form]["lists_per_s"] = out[form].pop("events_per_s")
-    out["speedup"] = round(out["batched_graph"]["lists_per_s"] / out["plain"]["lists_per_s"], 2)
-    out["speedup_learned"] = round(out["batched_learned_graph"]["lists_per_s"] / out["plain"]["lists_per_s"], 2)
-    ms = out["batched_graph"]["ms_per_step"]
-    out["batched_GB/s"] = round(runner.algorithmic_bytes() / ms / 1e6, 1)
+    log(f"config 4: {len(specs)} arrays, {sum(N for _, N, _ in sizes)} elements, {len(waves.waves)} waves")
+    for wave, arrays in waves:
+        realized += waves.bytes(wave)
+        log(f"config 4 wave: arrays {wave[0]}..{wave[-1]}, {waves.bytes(wave) / 2**30:.1f} GiB")
+        wrapped = [Array(C.ListOffsetArray(o, C.NumpyArray(x, be), be)) for o, x in arrays]  # device tensors as they are
+
+        def plain():
+            for a in wrapped:
+                ak.sum(a, axis=-1)
+                ak.min(a, axis=-1)
+                ak.argmin(a, axis=-1)
+                ak.argsort(a, axis=-1)
+
+        total["plain"] += timed(plain)
+        del wrapped
+        torch.cuda.empty_cache()
+        runner = BatchRunner(arrays, variants=tiles)
+        if any(g.name not in tiles for g in runner.groups):
+            # each dtype's tile autotuned on the first wave that holds it, kept for the rest
+            for name, t in runner.tune(reps=2).items():
+                tiles.setdefault(name, t["best"])
+            runner = BatchRunner(arrays, variants=tiles)
+        total["batched"] += timed(runner.run)
+        for k, i in enumerate(wave):
+            off, x = arrays[k]
+            if prefix_elems + x.shape[0] <= cpu_elements:
+                host_prefix.append((off.cpu().numpy(), x.cpu().numpy()))
+                prefix_elems += x.shape[0]
+            if x.dtype in (torch.float32, torch.float64) and len(float_samples) < 4 and x.shape[0] <= cpu_elements \
+                    and x.shape[0] > 0:
+                float_samples.append((off.cpu().numpy(), x.cpu().numpy(), runner.result(k)["sum"].cpu().numpy()))
+        del runner
+        lrunner = BatchRunner(arrays, variants="learned")
+        total["batched_learned"] += timed(lrunner.run)
+        for g in lrunner.groups:
+            learned_tiles.setdefault(g.name, g.chosen())
+        del lrunner
+    HOST["config4"] = host_prefix
+    HOST["config4_sample"] = (f"the first {len(host_prefix)} arrays of the same batch (host copies), "
+                              f"{prefix_elems} elements")
+    ACCURACY["batch"] = float_samples
+    n_lists = sum(n for n, _, _ in sizes)
+    n_elem = sum(N for _, N, _ in sizes)
+    out = {"arrays": len(specs), "lists": n_lists, "elements": n_elem, "waves": len(waves.waves),
+           "wave_budget_GB": budget_gb, "realized_device_bytes": realized,
+           "sample": "full size: list counts log-uniform in [1e3, 1e7], seed 4 (lengths and values drawn on the "
+                     "device per array); streamed in waves, every form timed on every wave",
+           "autotuned_tiles": tiles, "learned_tiles_first_wave_per_dtype": learned_tiles}
+    runner_bytes = 0
+    for n, N, dt in sizes:
+        s_ = dt.itemsize
+        runner_bytes += 8 * (n + 1) + s_ * N + n * (8 + s_ + 8 + 1) + 8 * (n + 1) + s_ * N + 8 * N
+    for form, secs in total.items():
+        out[form] = {"ms_per_step": round(secs * 1e3, 2), "lists_per_s": round(n_lists / secs, 1)}
+    out["speedup"] = round(total["plain"] / total["batched"], 2)
+    out["speedup_learned"] = round(total["plain"] / total["batched_learned"], 2)
+    out["batched_GB/s"] = round(runner_bytes / total["batched"] / 1e9, 1)
+    out["algorithmic_bytes"] = runner_bytes
     return out
 
 
@@ -467,29 +591,158 @@ def roofline_probe(args):
     print("roofline probe done", flush=True)
 
 
-def cpu_baseline(ak, off, fields, sample_events):
+HOST = {}  # host copies of the configs' inputs, for the CPU baselines
+
+
+def make_config1():
+    """Config 1: 1M lists, Poisson(10) lengths, int64 offsets, float64 N(0,1), seed 1."""
     import numpy as np
 
-    from oracle.backend import OracleBackend
+    rng = np.random.default_rng(1)
+    counts = rng.poisson(10, 1_000_000)
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    return off, rng.standard_normal(int(off[-1]))
 
-    be = OracleBackend()
-    n = min(sample_events, len(off) - 1)
-    end = int(off[n])
-    sub_off = off[: n + 1].copy()
-    sub_fields = {k: np.ascontiguousarray(v[:end]) for k, v in fields.items()}
-    ev = ak.from_offsets(sub_off, sub_fields, be)
-    pipeline(ak, ev)  # warm-up
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        pipeline(ak, ev)
-        reps += 1
-        if time.perf_counter() - t0 > 10.0 or reps >= 5:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": round(n / dt, 1), "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} events of the same workload, {reps} passes of the full pipeline "
-                      f"through the same host layer on the single-threaded C++ oracle"}
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class _OneCore:
+    """Pins this process to one core for the CPU timings (what taskset -c k
+    does), restoring the affinity afterwards."""
+
+    def __enter__(self):
+        self.prev = os.sched_getaffinity(0)
+        self.core = max(self.prev)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.prev)
+
+
+def _timed_oracle():
+    """The oracle backend with every registry call's C function timed
+    (time.perf_counter_ns): `be.kernel_ns` accumulates the kernel-only time
+    of a host-layer sequence."""
+    from oracle.backend import OracleBackend, OracleKernel
+
+    class Timed(OracleBackend):
+        kernel_ns = 0
+
+        def _make_kernel(self, name):
+            k = OracleKernel(name)
+            fn = k._fn
+            be = self
+
+            def timed(*a):
+                t = time.perf_counter_ns()
+                r = fn(*a)
+                be.kernel_ns += time.perf_counter_ns() - t
+                return r
+
+            k._fn = timed
+            return k
+
+    return Timed()
+
+
+def _median_runs(fn, be, reps=5):
+    """1 warm-up, then the median end-to-end and kernel-only milliseconds of
+    `reps` runs."""
+    fn()
+    e2e, kern = [], []
+    for _ in range(reps):
+        be.kernel_ns = 0
+        t = time.perf_counter_ns()
+        fn()
+        e2e.append(time.perf_counter_ns() - t)
+        kern.append(be.kernel_ns)
+    med = lambda v: sorted(v)[len(v) // 2] / 1e6
+    return med(e2e), med(kern)
+
+
+def cpu_baselines(ak, args, reps=5):
+    """The CPU baseline of every config on the GPU box's host: the oracle's
+    single-threaded C++ kernels (-O3, no OpenMP, like awkward-cpp's own
+    build) on host copies of the same seeded inputs, pinned to one core;
+    end-to-end (through the same host layer) and kernel-only (the C calls
+    alone) medians of `reps` runs after a warm-up."""
+    import numpy as np
+
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "cpu_model": _cpu_model(),
+            "cores_used": 1, "kind": "port", "reps": reps,
+            "timing": "1 warm-up, median of reps, time.perf_counter_ns; kernel_only = the oracle C calls alone, "
+                      "end_to_end = the same ak.* host layer over numpy buffers"}
+    out = {"host": info}
+    with _OneCore() as pin:
+        info["pinned_core"] = pin.core
+        # config 1: the segmented reductions, one call each
+        off, x = HOST.get("config1") or make_config1()
+        n, N = len(off) - 1, len(x)
+        be = _timed_oracle()
+        arr = ak.from_offsets(off, x, be)
+        c1 = {"lists": n, "elements": N}
+        nbytes = {"sum": 8 * (n + 1) + 8 * N + 8 * n, "max": 8 * (n + 1) + 8 * N + 8 * n,
+                  "argmax": 8 * (n + 1) + 8 * N + 9 * n, "count": 8 * (n + 1) + 8 * n}
+        for op in ("sum", "max", "argmax", "count"):
+            f = getattr(ak, op)
+            e2e, kern = _median_runs(lambda: f(arr, axis=-1), be, reps)
+            c1[op] = {"end_to_end_ms": round(e2e, 3), "kernel_only_ms": round(kern, 3),
+                      "kernel_GB/s": round(nbytes[op] / (kern * 1e6), 2) if kern > 0 else None,
+                      "bytes": nbytes[op]}
+        out["config1"] = c1
+        # config 2: the event pipeline on the first `cpu_sample` events
+        off2, fields2 = HOST["config2"]
+        k = min(args.cpu_sample, len(off2) - 1)
+        end = int(off2[k])
+        be = _timed_oracle()
+        ev = ak.from_offsets(off2[: k + 1].copy(), {f: np.ascontiguousarray(v[:end]) for f, v in fields2.items()}, be)
+        e2e, kern = _median_runs(lambda: pipeline(ak, ev), be, reps)
+        out["config2"] = {"events": k, "end_to_end_ms": round(e2e, 3), "kernel_only_ms": round(kern, 3),
+                          "events_per_s": round(k / (e2e * 1e-3), 1),
+                          "sample": f"first {k} of the 50M events (the full workload at its seed)"}
+        # config 3: the combinations step on the first 1M of the 20M events
+        if "config3" in HOST:
+            off3, fields3 = HOST["config3"]
+            k = min(1_000_000, len(off3) - 1)
+            end = int(off3[k])
+            be = _timed_oracle()
+            ev3 = ak.from_offsets(off3[: k + 1].copy(), {f: np.ascontiguousarray(v[:end]) for f, v in fields3.items()},
+                                  be)
+            e2e, kern = _median_runs(lambda: pairs_step(ak, np, ev3), be, reps)
+            out["config3"] = {"events": k, "end_to_end_ms": round(e2e, 3), "kernel_only_ms": round(kern, 3),
+                              "events_per_s": round(k / (e2e * 1e-3), 1),
+                              "sample": f"first {k} of the 20M events (Poisson(4) clipped [0,30], seed 3)"}
+        # config 4: per array sum, min, argmin, argsort on a bounded prefix of the batch
+        if "config4" in HOST:
+            arrays = HOST["config4"]
+            be = _timed_oracle()
+            host = [ak.from_offsets(o, v, be) for o, v in arrays]
+
+            def batch():
+                for a in host:
+                    ak.sum(a, axis=-1)
+                    ak.min(a, axis=-1)
+                    ak.argmin(a, axis=-1)
+                    ak.argsort(a, axis=-1)
+
+            e2e, kern = _median_runs(batch, be, reps)
+            lists = sum(len(o) - 1 for o, _ in arrays)
+            out["config4"] = {"arrays": len(arrays), "lists": lists, "elements": int(sum(len(v) for _, v in arrays)),
+                              "end_to_end_ms": round(e2e, 3), "kernel_only_ms": round(kern, 3),
+                              "lists_per_s": round(lists / (e2e * 1e-3), 1),
+                              "sample": HOST.get("config4_sample", "")}
+    return out
 
 
 ACCURACY = {}  # host copies of inputs and GPU sums for the fsum report
@@ -551,6 +804,113 @@ def accuracy_report(ak, off, fields, gpu_sum, sample_events):
     return report
 
 
+def _fsum_lists(off, x, limit=20_000_000):
+    """math.fsum of each list, over the first lists holding at most `limit`
+    elements; returns (lists used, exact sums)."""
+    import math
+
+    import numpy as np
+
+    k = int(np.searchsorted(off, limit, side="right")) - 1
+    k = max(1, min(k, len(off) - 1))
+    return k, np.array([math.fsum(x[a:b]) for a, b in zip(off[:k], off[1 : k + 1])])
+
+
+def _oracle_sum(off, x):
+    import numpy as np
+
+    import oracle
+
+    dt = np.dtype(x.dtype).name
+    out = np.empty(len(off) - 1, dtype=x.dtype)
+    err = oracle.kernel(f"awkward_reduce_sum_{dt}_{dt}_64")(out.ctypes.data, x.ctypes.data, off.ctypes.data,
+                                                             len(off) - 1)
+    assert err.str is None
+    return out
+
+
+def _accuracy_entry(off, x, gpu, note):
+    import numpy as np
+
+    k, exact = _fsum_lists(off, x)
+    cpu = _oracle_sum(off, x)
+    w = np.int32 if x.dtype == np.float32 else np.int64
+    o = off[: k + 1]
+    return {"lists": int(k), "elements": int(off[k] - off[0]), "dtype": np.dtype(x.dtype).name, "path": note,
+            "gpu": _fsum_errors(o, x, gpu[:k], exact), "cpu": _fsum_errors(o, x, cpu[:k], exact),
+            "gpu_bits_equal_cpu": float(np.mean(gpu[:k].view(w) == cpu[:k].view(w)))}
+
+
+def accuracy_reordered(torch):
+    """fsum errors of the float sums whose order the GPU changes: lists of
+    33..400 elements (the heavy tile: partials per wave, then combined),
+    32 lists of 500,000 elements through the long-list queue (variant 10:
+    chunk partials combined in order), and the batch runner's float
+    arrays; for float32 and float64, GPU and CPU against the same fsum."""
+    import numpy as np
+
+    from awkward_amd import dispatch
+
+    rng = np.random.default_rng(5)
+    out = {}
+    for dt in (np.float32, np.float64):
+        for name, counts, variant in (("heavy_tile_lists_33_to_400", rng.integers(33, 401, 60_000), None),
+                                      ("long_list_queue", np.full(32, 500_000), dispatch.LONG_LISTS)):
+            off = np.zeros(len(counts) + 1, dtype=np.int64)
+            np.cumsum(counts, out=off[1:])
+            x = rng.standard_normal(int(off[-1])).astype(dt)
+            n = len(counts)
+            v = dispatch.heuristic_variant(n, len(x), x.itemsize) if variant is None else variant
+            res = torch.empty(n, dtype=torch.float32 if dt == np.float32 else torch.float64, device="cuda")
+            err = dispatch.reduce_sum(res, torch.from_numpy(x).cuda(), torch.from_numpy(off).cuda(), n, v)
+            assert err.str is None
+            out[f"{name}_{np.dtype(dt).name}"] = _accuracy_entry(off, x, res.cpu().numpy(), f"tuned sum, variant {v}")
+    for i, (off, x, gpu) in enumerate(ACCURACY.get("batch", [])):
+        out[f"batch_array_{i}_{np.dtype(x.dtype).name}"] = _accuracy_entry(off, x, gpu, "batch runner reduce")
+    return out
+
+
+def headline_vs_oracle(ak, off, fields, gpu_out, sample_events):
+    """num and argmax of the fused headline run against the oracle's
+    pipeline on the first `sample_events` events: bit-equal fractions."""
+    import numpy as np
+
+    from oracle.backend import OracleBackend
+
+    n = min(sample_events, len(off) - 1)
+    end = int(off[n])
+    sub = ak.from_offsets(off[: n + 1].copy(), {k: np.ascontiguousarray(v[:end]) for k, v in fields.items()},
+                          OracleBackend())
+    num, _, am = pipeline(ak, sub)
+    cn = np.asarray(num.layout.data)
+    cam, cmask = np.asarray(am.layout.content.data), np.asarray(am.layout.mask).astype(np.int8)
+    gn, gam, gmask = gpu_out
+    valid = cmask == 0
+    return {"events": n, "num_equal": float(np.mean(gn[:n] == cn)),
+            "argmax_mask_equal": float(np.mean(gmask[:n] == cmask)),
+            "argmax_equal_where_valid": float(np.mean(gam[:n][valid] == cam[valid])) if valid.any() else 1.0}
+
+
+def _section(name, fn, *a):
+    """A bench section whose failure is reported in its entry instead of
+    ending the run (the headline line is still printed)."""
+    import traceback
+
+    try:
+        return fn(*a)
+    except Exception as e:  # noqa: BLE001 -- reported, not hidden
+        log(f"{name} failed: {e!r}")
+        traceback.print_exc()
+        import gc
+
+        import torch
+
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return {"error": repr(e)}
+
+
 def time_form(torch, fn, steps, warmup, events):
     for _ in range(warmup):
         fn()
@@ -599,6 +959,7 @@ def main():
 
     log(f"generating {args.events} events")
     off, fields = make_events(args.events)
+    HOST["config2"] = (off, fields)
     plain_be = HipBackend(fused=False)
     tuned_be = HipBackend(fused=True)
     ev_plain = ak.from_offsets(off, fields, plain_be)
@@ -640,6 +1001,7 @@ def main():
     log(f"fused_graph: {forms['fused_graph']}")
 
     gpu_sum_pt = pipe.out["sum"][: args.cpu_sample].cpu().numpy()
+    gpu_num_argmax = tuple(pipe.out[k][: args.cpu_sample].cpu().numpy() for k in ("num", "argmax", "argmax_mask"))
     alg_bytes = pipe.algorithmic_bytes(N)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -650,20 +1012,29 @@ def main():
     reductions = None if args.no_reductions else run_segmented_reductions(ak, tuned_be, torch)
     del ev, ev_plain, pipe, graph
     torch.cuda.empty_cache()
-    combos = None if args.no_combinations else run_combinations(ak, tuned_be, torch)
-    if combos:
+    combos = None if args.no_combinations else _section("combinations", run_combinations, ak, tuned_be, torch)
+    if combos and "error" not in combos:
         log(f"combinations: plain {combos['plain']['events_per_s']:.4g} ev/s, fused {combos['fused_graph']['events_per_s']:.4g} ev/s")
     torch.cuda.empty_cache()
-    batched = None if args.no_batched else run_batched(ak, torch)
-    if batched:
-        log(f"batched: plain {batched['plain']['lists_per_s']:.4g} lists/s, batched {batched['batched_graph']['lists_per_s']:.4g} lists/s")
+    batched = None if args.no_batched else _section("batched", run_batched, ak, torch)
+    if batched and "error" not in batched:
+        log(f"batched: plain {batched['plain']['lists_per_s']:.4g} lists/s, batched {batched['batched']['lists_per_s']:.4g} lists/s")
     regret = None if args.no_dispatch else run_dispatch_regret(torch)
     if regret:
         log(f"dispatch: learned regret {regret['learned']['aggregate']}, table {regret['table']['aggregate']}, "
             f"heuristic {regret['heuristic']['aggregate']} over {regret['rows']} held-out rows")
-    cpu = None if args.no_cpu else cpu_baseline(ak, off, fields, args.cpu_sample)
-    accuracy = None if args.no_cpu else accuracy_report(ak, off, fields, gpu_sum_pt, args.cpu_sample)
-    if accuracy:
+    cpu, accuracy = None, None
+    if not args.no_cpu:
+        per_config = cpu_baselines(ak, args)
+        log(f"cpu baselines: {json.dumps(per_config)}")
+        c2 = per_config["config2"]
+        cpu = {"value": c2["events_per_s"], "unit": "events/s", "cores": 1, "kind": "port",
+               "sample": c2["sample"] + "; end-to-end through the same host layer on the single-threaded C++ oracle, "
+                                        "pinned to one core", "per_config": per_config}
+        accuracy = accuracy_report(ak, off, fields, gpu_sum_pt, args.cpu_sample)
+        accuracy["reordered"] = accuracy_reordered(torch)
+        accuracy["headline_num_argmax_vs_oracle"] = headline_vs_oracle(ak, off, fields, gpu_num_argmax,
+                                                                       args.cpu_sample)
         log(f"accuracy: {json.dumps(accuracy)}")
 
     line = {

@@ -25,8 +25,10 @@ extern "C" {
    pools, so work on different streams never shares scratch or errors.  A
    pool that grows retires its old buffer instead of freeing it (in-flight
    kernels and captured hipGraphs may still use it); retired buffers stay
-   allocated until the process ends.  No entry point synchronises the device
-   or frees memory to grow a pool. */
+   allocated until the process ends or awkward_hip_release_pools frees
+   them.  Pools double up to 1 GiB and then grow to the request (rounded to
+   64 MiB).  No entry point synchronises the device or frees memory to grow
+   a pool. */
 /* Grow the current stream's scratch pool to at least `bytes` now (and create
    the stream's state), e.g. before capturing a hipGraph on that stream.
    0 on success. */
@@ -36,6 +38,11 @@ int awkward_hip_reserve_queue(int64_t bytes);
 /* Diagnostics: out[0] = bytes of the current stream's scratch pool, out[1] =
    of its queue pool, out[2] = number of retired pool buffers. */
 void awkward_hip_pool_bytes(int64_t* out);
+/* Synchronise the current stream and free its pools' buffers, current and
+   retired (the next call allocates afresh); only when no captured hipGraph
+   still uses them (no counterpart in the reference: for callers that stream
+   batches larger than HBM).  0 on success. */
+int awkward_hip_release_pools(void);
 /* Number of reduce tile variants (threads x lists per thread x LDS stage). */
 int awkward_hip_reduce_variant_count(void);
 

@@ -126,3 +126,30 @@ def test_batch_long_lists():
     torch.cuda.synchronize()
     for i, (o, x) in enumerate(batch):
         check_array(o, x, runner.result(i))
+
+
+def test_waves_straddle_the_budget():
+    """A batch run in waves (a budget that splits it into several runs of
+    arrays, one array alone because it exceeds the budget): every array's
+    results equal the oracle's, as in one run."""
+    import torch
+
+    from awkward_amd.batch import BatchRunner, WaveRunner, device_bytes
+
+    batch = make_batch(77, narrays=14)
+    sizes = [device_bytes(len(o) - 1, len(x), x.itemsize) for o, x in batch]
+    budget = sorted(sizes)[len(sizes) // 2] * 2
+    specs = [(len(o) - 1, len(x), x.dtype) for o, x in batch]
+    waves = WaveRunner(specs, lambda i: (torch.from_numpy(batch[i][0]).cuda(), torch.from_numpy(batch[i][1]).cuda()),
+                       budget)
+    assert len(waves.waves) >= 3 and sorted(i for w in waves.waves for i in w) == list(range(len(batch)))
+    assert all(waves.bytes(w) <= budget or len(w) == 1 for w in waves.waves)
+    seen = 0
+    for wave, arrays in waves:
+        runner = BatchRunner(arrays)
+        runner.run()
+        torch.cuda.synchronize()
+        for k, i in enumerate(wave):
+            check_array(batch[i][0], batch[i][1], runner.result(k))
+            seen += 1
+    assert seen == len(batch)
