@@ -668,10 +668,35 @@ class ListArray(Content):
         return ListOffsetArray(nextoffsets, self.content._carry(nextcarry, True), self.backend)
 
     # -- jagged integer index, NumpyArray slice content: listarray.py:549-618 --
-    def _getitem_next_jagged(self, slicestarts, slicestops, sliceindex):
+    def _getitem_next_jagged(self, slicestarts, slicestops, slicecontent):
+        """self sliced list by list by [slicestarts[i], slicestops[i]) of
+        slicecontent: an int64 NumpyArray (jagged_carrylen + jagged_apply),
+        or lists of a deeper slice (listarray.py:489-547: jagged_descend
+        checks every list's length, then the content is sliced by the
+        slice's inner lists)."""
         nplike = self.nplike
         starts, stops = self._contig()
         cdt = nplike.dtype(starts)
+        if isinstance(slicecontent, ListOffsetArray):
+            n = nplike.length(slicestarts)
+            outoffsets = nplike.empty(n + 1, np.int64)
+            err = self._kernel("awkward_ListArray_getitem_jagged_descend", np.int64, np.int64, np.int64, cdt, cdt)(
+                outoffsets, slicestarts, slicestops, n, starts, stops)
+            if err.str is not None:
+                raise IndexError(self.backend.format_kernel_error(err))
+            compact = self.to_ListOffsetArray64()
+            offs = compact.offsets
+            first, last = nplike.item(offs, 0), nplike.item(offs, nplike.length(offs) - 1)
+            next_content = compact.content.slice(first, last)
+            if not next_content.is_list:
+                raise IndexError("too many jagged slice dimensions for array")
+            if isinstance(next_content, ListOffsetArray):
+                next_content = next_content.to_ListArray()
+            so = slicecontent.offsets
+            m = nplike.length(so)
+            outcontent = next_content._getitem_next_jagged(so[: m - 1], so[1:], slicecontent.content)
+            return ListOffsetArray(outoffsets, outcontent, self.backend)
+        sliceindex = slicecontent.data
         carrylen_buf = nplike.empty(1, np.int64)
         self._check(
             self._kernel("awkward_ListArray_getitem_jagged_carrylen", np.int64, np.int64, np.int64)(
@@ -706,7 +731,7 @@ class ListArray(Content):
                 multistarts, multistops, item.offsets, item.length, 1
             )
         )
-        return self._getitem_next_jagged(multistarts, multistops, item.content.data)
+        return self._getitem_next_jagged(multistarts, multistops, item.content)
 
     @property
     def depth(self):

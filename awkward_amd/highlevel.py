@@ -211,10 +211,16 @@ def _broadcast_ufunc(ufunc, inputs):
 
 
 def _getitem_jagged(layout, item):
-    """layout[item] for a jagged boolean or integer item with one list level
-    (Content._getitem, content.py:561-656; _slicing.py:476-530)."""
+    """layout[item] for a jagged boolean or integer item, one or more list
+    levels deep (Content._getitem, content.py:561-656; the slice normalised
+    as _slicing.py:476-530 does; nested levels through
+    ListArray._getitem_next_jagged's descend, listarray.py:489-547).  A
+    record array is sliced field by field."""
+    if isinstance(layout, C.RecordArray):
+        return C.RecordArray([_getitem_jagged(c, item) for c in layout.contents], layout.fields, layout.length,
+                             layout.backend)
     if not layout.is_list:
-        raise NotImplementedError("jagged getitem needs a list array")
+        raise IndexError("too many jagged slice dimensions for array")
     if isinstance(item, C.ListArray):
         item = item.to_ListOffsetArray64()
     if not isinstance(item, C.ListOffsetArray):
@@ -227,20 +233,41 @@ def _getitem_jagged(layout, item):
     offsets = item.offsets
     if nplike.dtype(offsets) != np.dtype(np.int64):
         offsets = nplike.asarray(offsets, np.int64)
-    if content.dtype == np.dtype(np.bool_):
+    if not content.is_list and content.dtype == np.dtype(np.bool_):
         fused = _mask_getitem(layout, offsets, content)
         if fused is not None:
             return fused
-        index, nextoffsets = nplike.bool_to_jagged_index(offsets, content.data)
-        item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
-    else:
-        nextoffsets, trimmed = item._zero_based()
-        index = trimmed.data
-        if nplike.dtype(index) != np.dtype(np.int64):
-            index = nplike.asarray(index, np.int64)
-        item = C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
+    item = _jagged_index(C.ListOffsetArray(offsets, content, backend))
     target = layout.to_ListArray() if isinstance(layout, C.ListOffsetArray) else layout
     return target._getitem_jagged(item)
+
+
+def _jagged_index(item):
+    """A jagged slice as the kernels take it: int64 offsets from 0 at every
+    level over an int64 NumpyArray; boolean leaves become the local
+    positions of their true values (_slicing._normalise_item_bool_to_int)."""
+    backend = item.backend
+    nplike = backend.nplike
+    content = item.content
+    if isinstance(content, C.IndexedArray):
+        content = content.project()
+    if isinstance(content, (C.ListArray, C.ListOffsetArray)):
+        if isinstance(content, C.ListArray):
+            content = content.to_ListOffsetArray64()
+        nextoffsets, trimmed = C.ListOffsetArray(item.offsets, content, backend)._zero_based()
+        return C.ListOffsetArray(nextoffsets, _jagged_index(trimmed), backend)
+    if not isinstance(content, C.NumpyArray):
+        raise NotImplementedError("jagged getitem needs integer or boolean lists")
+    if content.dtype == np.dtype(np.bool_):
+        index, nextoffsets = nplike.bool_to_jagged_index(item.offsets, content.data)
+        return C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
+    if content.dtype.kind not in "iu" and content.length > 0:  # an empty slice is an empty int64 index
+        raise IndexError("jagged slices must hold integers or booleans")
+    nextoffsets, trimmed = C.ListOffsetArray(item.offsets, content, backend)._zero_based()
+    index = trimmed.data
+    if nplike.dtype(index) != np.dtype(np.int64):
+        index = nplike.asarray(index, np.int64)
+    return C.ListOffsetArray(nextoffsets, C.NumpyArray(index, backend), backend)
 
 
 def _mask_getitem(layout, moffsets, mask):
@@ -467,8 +494,7 @@ def _layout_from_python(items, backend, dtype):
         return C.ListOffsetArray(nplike.asarray(offsets), inner, backend)
     if items and isinstance(items[0], dict):
         keys = list(items[0])
-        cols = [C.NumpyArray(nplike.asarray(np.ascontiguousarray(np.asarray([r[k] for r in items], dtype=dtype))),
-                             backend) for k in keys]
+        cols = [_layout_from_python([r[k] for r in items], backend, dtype) for k in keys]
         return C.RecordArray(cols, keys, len(items), backend)
     values = np.asarray(items, dtype=dtype if dtype is not None else (np.float64 if not items else None))
     return C.NumpyArray(nplike.asarray(np.ascontiguousarray(values)), backend)
