@@ -1,9 +1,14 @@
 """Builds libawkward_hip.so (gfx950) from awkward_amd/csrc/*.hip with hipcc.
 
-    python -m awkward_amd.build          # or awkward_amd.build.build()
+    python -m awkward_amd.build            # or awkward_amd.build.build()
+    python -m awkward_amd.build sanitize   # host code under ASan + UBSan
 
 Objects go to awkward_amd/_build/, the library to awkward_amd/lib/.  Every
-source compiles in parallel; the whole build takes about a minute.
+source compiles in parallel; the whole build takes about a minute.  The
+sanitized build instruments the host code only (each sanitizer flag after
+-Xarch_host: device code is compiled as usual) into
+awkward_amd/lib/san/libawkward_hip.so; load it with AWKWARD_HIP_LIBRARY and
+clang's ASan runtime (SAN_RUNTIME) preloaded.
 """
 
 from __future__ import annotations
@@ -37,31 +42,54 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJDIR, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+#: host-only sanitizers for the sanitized build
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Xarch_host",
+             "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+SAN_OBJDIR = os.path.join(HERE, "_build_san")
+SAN_LIBPATH = os.path.join(LIBDIR, "san", "libawkward_hip.so")
+
+
+def san_runtime() -> str:
+    """The ASan runtime of hipcc's host compiler (the process must preload it)."""
+    found = sorted(glob.glob(os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "llvm", "lib", "clang", "*",
+                                          "lib", "linux", "libclang_rt.asan-x86_64.so")))
+    if not found:
+        raise FileNotFoundError("clang's ASan runtime is not installed")
+    return found[-1]
+
+
+def _compile(src: str, objdir: str = OBJDIR, extra=()) -> str:
+    obj = os.path.join(objdir, os.path.basename(src).replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{proc.stderr}")
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(verbose: bool = True, sanitize: bool = False) -> str:
+    objdir, libpath = (SAN_OBJDIR, SAN_LIBPATH) if sanitize else (OBJDIR, LIBPATH)
+    extra = SAN_FLAGS if sanitize else ()
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(os.path.dirname(libpath), exist_ok=True)
     sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
-        objects = list(pool.map(_compile, sources))
-    tmp = LIBPATH + ".tmp"
+        objects = list(pool.map(lambda src: _compile(src, objdir, extra), sources))
+    tmp = libpath + ".tmp"
+    # the sanitized library takes its runtime from the preloaded ASan runtime
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objects, "-o", tmp]
+    if sanitize:
+        cmd[3:3] = ["-Wl,--allow-shlib-undefined", "-Wl,--unresolved-symbols=ignore-all"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
-    os.replace(tmp, LIBPATH)
+    os.replace(tmp, libpath)
     if verbose:
-        print(f"built {LIBPATH} from {len(sources)} sources for {ARCH}")
-    return LIBPATH
+        print(f"built {libpath} from {len(sources)} sources for {ARCH}")
+    return libpath
 
 
 if __name__ == "__main__":
-    build()
+    import sys
+
+    build(sanitize=len(sys.argv) > 1 and sys.argv[1] == "sanitize")

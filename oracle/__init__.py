@@ -30,7 +30,8 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "liboracle.so")
+#: ORACLE_LIB selects another build of the same kernels (the sanitized one)
+LIBPATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class ERROR(ctypes.Structure):
