@@ -100,6 +100,166 @@ __global__ __launch_bounds__(kEvalBlock) void expr_bits_kernel(ExprProgram p, ui
   }
 }
 
+// ---- selection cuts -------------------------------------------------------------
+// A predicate that is a conjunction of comparisons of float32 columns (or of
+// their absolute values) with constants -- a selection cut such as
+// (pt > 25) & (abs(eta) < 2.4) -- needs no interpreter: cut_bits_kernel
+// evaluates the same float32 comparisons (so the same bits) with each lane's
+// loads of every term in flight together.
+constexpr int kMaxCutTerms = 8;
+
+struct CutTerm {
+  const float* col;
+  float c;
+  int op;  // kXGt .. kXNe
+  int absolute;
+};
+
+struct Cut {
+  CutTerm t[kMaxCutTerms];
+  int n;
+};
+
+// A symbolic run of the program: a register holds a float32 column (a leaf
+// read without an index), its absolute value, or a conjunction of terms (a
+// set of cut.t); any other op, type or operand means the program is not a cut.
+inline bool match_cut(const ExprProgram& p, Cut* cut) {
+  if (p.out_type != kTBool || p.nops < 1 || p.wide) return false;
+  enum Kind { kNone = 0, kCol, kAbs, kTerms };
+  int kind[kExprRegs] = {}, col[kExprRegs] = {};
+  uint32_t set[kExprRegs] = {};
+  for (int k = 0; k < p.nleaves; k++)
+    if (p.leaf[k].type == kTFloat32 && p.leaf[k].index == nullptr) {
+      kind[k] = kCol;
+      col[k] = k;
+    }
+  Cut all{};
+  for (int i = 0; i < p.nops; i++) {
+    const ExprOp& o = p.op[i];
+    int nk = kNone, nc = 0;
+    uint32_t ns = 0;
+    switch (o.code) {
+      case kXAbs:
+        if (o.type == kTFloat32 && kind[o.a] == kCol) {
+          nk = kAbs;
+          nc = col[o.a];
+        }
+        break;
+      case kXGt:
+      case kXGe:
+      case kXLt:
+      case kXLe:
+      case kXEq:
+      case kXNe:
+        if (o.type == kTFloat32 && o.bimm && o.to == 0 && (kind[o.a] == kCol || kind[o.a] == kAbs) &&
+            all.n < kMaxCutTerms) {
+          CutTerm& t = all.t[all.n];
+          t.col = (const float*)p.leaf[col[o.a]].data;
+          const uint32_t b = (uint32_t)o.imm;
+          __builtin_memcpy(&t.c, &b, sizeof(float));
+          t.op = o.code;
+          t.absolute = kind[o.a] == kAbs;
+          nk = kTerms;
+          ns = 1u << all.n;
+          all.n++;
+        }
+        break;
+      case kXAnd:
+      case kXLogicalAnd:
+      case kXMul:  // bool * bool is &
+        if (o.type == kTBool && !o.bimm && kind[o.a] == kTerms && kind[o.b] == kTerms) {
+          nk = kTerms;
+          ns = set[o.a] | set[o.b];
+        }
+        break;
+      default:
+        break;
+    }
+    if (nk == kNone) return false;
+    kind[o.dst] = nk;
+    col[o.dst] = nc;
+    set[o.dst] = ns;
+  }
+  if (kind[p.out] != kTerms) return false;
+  cut->n = 0;
+  for (int k = 0; k < all.n; k++)
+    if (set[p.out] & (1u << k)) cut->t[cut->n++] = all.t[k];
+  return cut->n > 0;
+}
+
+__device__ inline bool cut_compare(float x, int op, float c) {
+  switch (op) {
+    case kXGt: return x > c;
+    case kXGe: return x >= c;
+    case kXLt: return x < c;
+    case kXLe: return x <= c;
+    case kXEq: return x == c;
+    default: return x != c;
+  }
+}
+
+// Bits of a cut with NT terms: each wave takes groups of 64 * U consecutive
+// elements, lane l elements base + 64 u + l (every load coalesced, all NT * U
+// of a lane in flight at once), and stores one ballot word per 64 elements.
+constexpr int kCutBlock = 256;
+constexpr int kCutU = 8;
+
+template <int NT>
+__global__ __launch_bounds__(kCutBlock) void cut_bits_kernel(Cut cut, uint64_t* __restrict__ bits, int64_t n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nwaves = (int64_t)gridDim.x * (kCutBlock / kWave);
+  const float* col[NT];
+  float c[NT];
+  int op[NT], ab[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    col[t] = cut.t[t].col;
+    c[t] = cut.t[t].c;
+    op[t] = cut.t[t].op;
+    ab[t] = cut.t[t].absolute;
+  }
+  for (int64_t g = (int64_t)blockIdx.x * (kCutBlock / kWave) + threadIdx.x / kWave; g * (kWave * kCutU) < n;
+       g += nwaves) {
+    const int64_t base = g * (kWave * kCutU);
+    float x[NT][kCutU];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int u = 0; u < kCutU; u++) {
+        const int64_t i = base + u * kWave + lane;
+        x[t][u] = i < n ? col[t][i] : 0.0f;
+      }
+#pragma unroll
+    for (int u = 0; u < kCutU; u++) {
+      const int64_t first = base + u * kWave;
+      bool keep = first + lane < n;
+#pragma unroll
+      for (int t = 0; t < NT; t++) keep = keep && cut_compare(ab[t] ? fabsf(x[t][u]) : x[t][u], op[t], c[t]);
+      const uint64_t word = __ballot(keep);
+      if (lane == 0 && first < n) bits[first >> 6] = word;
+    }
+  }
+}
+
+inline void launch_cut_bits(const Cut& cut, uint64_t* bits, int64_t n, hipStream_t st) {
+  int64_t grid = ceil_div(n, (int64_t)kCutBlock * kCutU);
+  const int64_t cap = (int64_t)cu_count() * 32;
+  if (grid > cap) grid = cap;
+  switch (cut.n) {
+#define CUT_CASE(K) \
+  case K: cut_bits_kernel<K><<<(unsigned)grid, kCutBlock, 0, st>>>(cut, bits, n); break;
+    CUT_CASE(1)
+    CUT_CASE(2)
+    CUT_CASE(3)
+    CUT_CASE(4)
+    CUT_CASE(5)
+    CUT_CASE(6)
+    CUT_CASE(7)
+    default: cut_bits_kernel<8><<<(unsigned)grid, kCutBlock, 0, st>>>(cut, bits, n); break;
+#undef CUT_CASE
+  }
+}
+
 __device__ inline bool bit_at(const uint64_t* bits, int64_t j) { return (bits[j >> 6] >> (j & 63)) & 1; }
 
 // true bits in [a, b)
@@ -247,6 +407,13 @@ Error awkward_hip_expr_eval(void* out, const ExprProgram* program, int64_t start
   return success();
 }
 
+// The number of comparison terms of a program that is a selection cut, else 0
+// (host only).
+int awkward_hip_expr_cut_terms(const ExprProgram* program) {
+  Cut cut;
+  return (expr_program_ok(*program) && match_cut(*program, &cut)) ? cut.n : 0;
+}
+
 // array[mask] for a jagged boolean mask with int64 offsets `moffsets` (n
 // lists, absolute positions into the program's masklen elements): tooffsets (n + 1,
 // from 0) and tocarry (as many entries as the mask has true values; the
@@ -276,7 +443,10 @@ Error awkward_hip_mask_getitem(int64_t* tooffsets, int64_t* tocarry, const int64
   AKHIP_ERROR_WORD(ew);
   const uint32_t inv = next_invocation("awkward_ListArray_getitem_jagged_apply", kMaskMsgs);
   const ScanScratch s = scan_scratch_at(base + bits_bytes);
-  if (elements > 0) {
+  Cut cut;
+  if (elements > 0 && match_cut(p, &cut)) {
+    launch_cut_bits(cut, bits, elements, st);
+  } else if (elements > 0) {
     expr_dispatch(p, [&](auto kind) {
       using W = typename decltype(kind)::Word;
       constexpr int U = decltype(kind)::slots;

@@ -221,8 +221,13 @@ struct Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* toof
      the reference's _normalise_item_bool_to_int + jagged_expand + carrylen
      + jagged_apply sequence in one scan.  tocarry must hold the mask's
      true count (an upper bound such as its element count will do); the
-     count is tooffsets[n]. */
+     count is tooffsets[n].  A program that is a conjunction of float32
+     column comparisons (optionally of abs) with constants -- a selection
+     cut -- takes a dedicated kernel for its bits (the same comparisons).
+   expr_cut_terms: host only, no device work: the number of comparison
+     terms when the program is such a cut, else 0. */
 struct Error awkward_hip_expr_eval(void* out, const void* program, int64_t start, int64_t n);
+int awkward_hip_expr_cut_terms(const void* program);
 struct Error awkward_hip_mask_getitem(int64_t* tooffsets, int64_t* tocarry, const int64_t* fromstarts, const int64_t* fromstops, int64_t contentlen, const int64_t* moffsets, int64_t n, int64_t masklen, const void* program);
 
 /* ---- gathers and small host-layer steps ------------------------------------

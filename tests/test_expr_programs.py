@@ -168,3 +168,43 @@ def test_too_many_registers_raise(cols):
     prog, _ = half.program()
     valid(prog)
     assert prog.nregs > 8
+
+
+def test_selection_cuts_recognised(cols):
+    """The library's cut matcher (host code, no device work) takes exactly
+    the conjunctions of float32 column comparisons with constants, through
+    abs or not; anything else stays with the expression interpreter."""
+    import ctypes
+
+    from awkward_amd._lib import lib
+
+    f = lib().awkward_hip_expr_cut_terms
+    f.argtypes = [ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    a, t, by_ptr = cols
+    pt, eta, phi, q = (leafx(t[k]) for k in ("pt", "eta", "phi", "q"))
+
+    def terms(expr):
+        prog, _ = expr.program()
+        return f(ctypes.addressof(prog))
+
+    gt, lt, ab, band = np.greater, np.less, np.absolute, np.bitwise_and
+    assert terms(build(gt, pt, 25)) == 1
+    assert terms(build(band, build(gt, pt, 25), build(lt, build(ab, eta), 2.4))) == 2
+    assert terms(build(np.logical_and, build(lt, 0.5, pt), build(np.not_equal, phi, 0))) == 2
+    eight = build(gt, pt, 0)
+    for k in range(7):
+        eight = build(band, eight, build(lt, [pt, eta, phi][k % 3], 10 + k))
+    assert terms(eight) == 8
+    nine = build(band, eight, build(gt, phi, -9))
+    assert terms(nine) == 0  # more terms than the cut kernel takes
+    assert terms(build(np.bitwise_or, build(gt, pt, 25), build(lt, eta, 1))) == 0
+    assert terms(build(np.invert, build(gt, pt, 25))) == 0
+    assert terms(build(gt, build(np.add, pt, eta), 0.5)) == 0
+    assert terms(build(lt, pt, eta)) == 0  # column against column
+    assert terms(build(lt, q, 2)) == 0  # an int8 column
+    # a Python float is weak (NEP 50): compared in float32, against +inf here
+    assert terms(build(np.less, pt, 2.5e300)) == 1
+    d = leafx(torch.from_numpy(a["pt"].astype(np.float64)))
+    assert terms(build(lt, d, 1.0)) == 0  # a float64 column
+    assert terms(build(lt, pt, np.float64(1.0))) == 0  # a float64 scalar makes it a float64 comparison
