@@ -198,12 +198,16 @@ class RecordArray(Content):
 
 
 class IndexedArray(Content):
-    """A lazy carry: element i is content[index[i]]."""
+    """A lazy carry: element i is content[index[i]].  Buffers are immutable,
+    so a field's lazy carry and a projection are made once per node and
+    reused: ak.sum(sel.pt) then ak.argmax(sel.pt) gather sel.pt once."""
 
     def __init__(self, index, content, backend):
         self.index = index
         self.content = content
         self.backend = backend
+        self._fields = {}
+        self._projected = None
 
     @property
     def length(self):
@@ -213,16 +217,23 @@ class IndexedArray(Content):
         return f"IndexedArray(len={self.length}, {self.content!r})"
 
     def slice(self, start, stop):
+        if start == 0 and stop == self.length:
+            return self
         return IndexedArray(self.index[start:stop], self.content, self.backend)
 
     def _getitem_field(self, where):
-        return IndexedArray(self.index, self.content._getitem_field(where), self.backend)
+        field = self._fields.get(where)
+        if field is None:
+            field = self._fields[where] = IndexedArray(self.index, self.content._getitem_field(where), self.backend)
+        return field
 
     def _carry(self, carry, allow_lazy=True):
         return IndexedArray(self.nplike.take(self.index, carry), self.content, self.backend)
 
     def project(self):
-        return self.content._carry(self.index, False)
+        if self._projected is None:
+            self._projected = self.content._carry(self.index, False)
+        return self._projected
 
     def _reduce_next(self, reducer, starts, offsets, outlength, mask, keepdims, shifts=None):
         return self.project()._reduce_next(reducer, starts, offsets, outlength, mask, keepdims, shifts)
@@ -429,10 +440,14 @@ class RegularArray(Content):
 
 # ---- lists ------------------------------------------------------------------------
 class ListOffsetArray(Content):
-    def __init__(self, offsets, content, backend):
+    def __init__(self, offsets, content, backend, exact=False):
         self.offsets = offsets
         self.content = content
         self.backend = backend
+        # exact: offsets made by this library's own scans -- non-decreasing,
+        # from 0 to the content's length -- so no list selects more than the
+        # content holds (a range slice's carry fits in content.length)
+        self.exact = exact
 
     @property
     def is_list(self):
@@ -454,7 +469,7 @@ class ListOffsetArray(Content):
         return f"ListOffsetArray(len={self.length}, {self.content!r})"
 
     def _getitem_field(self, where):
-        return ListOffsetArray(self.offsets, self.content._getitem_field(where), self.backend)
+        return ListOffsetArray(self.offsets, self.content._getitem_field(where), self.backend, self.exact)
 
     def _carry(self, carry, allow_lazy=True):
         return ListArray(self.nplike.take(self.starts.contiguous() if hasattr(self.starts, "contiguous") else self.starts, carry),
@@ -471,6 +486,8 @@ class ListOffsetArray(Content):
         offsets = self.offsets
         if nplike.dtype(offsets) != np.dtype(np.int64):
             offsets = nplike.asarray(offsets, np.int64)
+        if self.exact:  # from 0 to the content's length: nothing to read or trim
+            return offsets, self.content
         first = nplike.item(offsets, 0)
         last = nplike.item(offsets, nplike.length(offsets) - 1)
         if first != 0:
@@ -603,7 +620,7 @@ class ListOffsetArray(Content):
         return ListOffsetArray(offsets, record, self.backend)
 
     def _getitem_range(self, start, stop, step):
-        return self.to_ListArray()._getitem_range(start, stop, step)
+        return self.to_ListArray()._getitem_range(start, stop, step, self.content.length if self.exact else None)
 
     def _getitem_jagged(self, item):
         return self.to_ListArray()._getitem_jagged(item)
@@ -642,7 +659,12 @@ class ListArray(Content):
         return ListArray(self.starts, self.stops, self.content._getitem_field(where), self.backend)
 
     # -- [:, start:stop:step]: listarray.py:782-870 --
-    def _getitem_range(self, start, stop, step):
+    def _getitem_range(self, start, stop, step, bound=None):
+        """bound: an upper bound of the carry's length known without a
+        device read (lists of exact offsets: their content's length).  With
+        it, and a nplike that fuses, the carry is allocated to the bound and
+        trimmed to next_range's own total, so the carrylength pass (and its
+        host read) is skipped; otherwise the reference's two kernels run."""
         nplike = self.nplike
         step = 1 if step is None else step
         if step == 0:
@@ -652,20 +674,26 @@ class ListArray(Content):
         starts, stops = self._contig()
         cdt = nplike.dtype(starts)
         lenstarts = self.length
-        carrylength = nplike.empty(1, np.int64)
-        self._check(
-            self._kernel("awkward_ListArray_getitem_next_range_carrylength", np.int64, cdt, cdt)(
-                carrylength, starts, stops, lenstarts, kstart, kstop, step
+        bounded = bound is not None and getattr(nplike, "fuse", False)
+        if bounded:
+            nextcarry = nplike.empty(max(bound, 1), np.int64)
+        else:
+            carrylength = nplike.empty(1, np.int64)
+            self._check(
+                self._kernel("awkward_ListArray_getitem_next_range_carrylength", np.int64, cdt, cdt)(
+                    carrylength, starts, stops, lenstarts, kstart, kstop, step
+                )
             )
-        )
-        nextcarry = nplike.empty(nplike.index_as_shape_item(carrylength), np.int64)
+            nextcarry = nplike.empty(nplike.index_as_shape_item(carrylength), np.int64)
         nextoffsets = nplike.empty(lenstarts + 1, cdt)
         self._check(
             self._kernel("awkward_ListArray_getitem_next_range", cdt, np.int64, cdt, cdt)(
                 nextoffsets, nextcarry, starts, stops, lenstarts, kstart, kstop, step
             )
         )
-        return ListOffsetArray(nextoffsets, self.content._carry(nextcarry, True), self.backend)
+        if bounded:
+            nextcarry = nextcarry[: nplike.item(nextoffsets, lenstarts)]
+        return ListOffsetArray(nextoffsets, self.content._carry(nextcarry, True), self.backend, exact=True)
 
     # -- jagged integer index, NumpyArray slice content: listarray.py:549-618 --
     def _getitem_next_jagged(self, slicestarts, slicestops, slicecontent):

@@ -292,7 +292,7 @@ def _mask_getitem(layout, moffsets, mask):
     if err.str is not None:
         raise IndexError(layout.backend.format_kernel_error(err))
     total = nplike.item(tooffsets, n)
-    return C.ListOffsetArray(tooffsets, target.content._carry(tocarry[:total], True), layout.backend)
+    return C.ListOffsetArray(tooffsets, target.content._carry(tocarry[:total], True), layout.backend, exact=True)
 
 
 # ---- axes ------------------------------------------------------------------------

@@ -323,3 +323,27 @@ def test_mask_getitem_cuts_match_oracle(k):
     assert got[2] == want[2]
     for g, w in zip(got[:2], want[:2]):
         same(np.array([x for lst in g for x in lst], np.float32), np.array([x for lst in w for x in lst], np.float32))
+
+
+@pytest.mark.parametrize("slc", [slice(1, None), slice(None, -1), slice(None, None, 2), slice(None, None, -1),
+                                 slice(-2, None), slice(1, 3), slice(5, 1, -2), slice(None, 0)])
+def test_mask_then_range_matches_oracle(slc):
+    """A range slice of a masked getitem's result (offsets made by the
+    library's own scan, from 0 to its content's length: the carry is
+    allocated to that bound and trimmed, with no carrylength pass), then
+    reductions of one field twice (its gather made once), against the
+    oracle backend's reference sequence."""
+    rng = np.random.default_rng(11)
+    nev = 5003
+    mult = np.clip(rng.poisson(4, nev), 0, 20).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(mult)]).astype(np.int64)
+    N = int(off[-1])
+    fields = {"pt": (5 + rng.exponential(20, N)).astype(np.float32), "eta": rng.uniform(-3, 3, N).astype(np.float32)}
+
+    def run(be):
+        ev = ak.from_offsets(off, fields, be)
+        sel = ev[(ev.pt > 15) & (abs(ev.eta) < 2.4)][:, slc]
+        return (ak.num(sel).to_list(), ak.sum(sel.pt, axis=-1).to_list(), ak.argmax(sel.pt, axis=-1).to_list(),
+                ak.max(sel.eta, axis=-1).to_list(), sel.eta.to_list())
+
+    assert run(hip()) == run(oracle())
