@@ -578,7 +578,7 @@ def main(argv=None):
     c.add_argument("--budget-s", type=float, default=None)
     c.add_argument("--out", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--data", required=True)
+    t.add_argument("--data", required=True, nargs="+", help="one or more collected .npz files (concatenated)")
     t.add_argument("--out", default=DEFAULT_MODEL)
     t.add_argument("--epochs", type=int, default=80)
     t.add_argument("--batch", type=int, default=512)
@@ -600,9 +600,11 @@ def main(argv=None):
                             n=np.array([s["n"] for s in specs]), N=np.array([s["N"] for s in specs]))
         log(f"wrote {args.out}: {X.shape[0]} rows")
     elif args.cmd == "train":
-        d = np.load(args.data)
+        parts = [np.load(path) for path in args.data]
+        X = np.concatenate([d["X"] for d in parts])
+        Tm = np.concatenate([d["times"] for d in parts])
         t0 = time.time()
-        model = train(d["X"], d["times"], device, epochs=args.epochs, bsz=args.batch, lr=args.lr, log=log)
+        model = train(X, Tm, device, epochs=args.epochs, bsz=args.batch, lr=args.lr, log=log)
         model.info["train_s"] = round(time.time() - t0, 2)
         model.save(args.out)
         log(json.dumps(model.info))

@@ -464,6 +464,9 @@ def run_dispatch_regret(torch, budget_s=40.0):
     sweep_s = time.perf_counter() - t0
     model = learned.Model.load(learned.DEFAULT_MODEL)
     out = learned.regret(model, X, T, device)
+    out["arrays"] = int(X.shape[0]) // 4
+    out["sample"] = ("held-out arrays (seeds 9000+) timed within the sweep's time budget; the full held-out set of "
+                     "10,000 arrays (40,000 rows) is evaluated offline by `python -m awkward_amd.learned evaluate`")
     out["sweep_s"] = round(sweep_s, 1)
     out["model"] = {k: model.info.get(k) for k in ("rows", "epochs", "val_regret", "train_s")}
     return out
