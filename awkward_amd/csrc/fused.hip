@@ -372,6 +372,195 @@ __global__ __launch_bounds__(BLOCK, 8) void event_pipeline_kernel(const int64_t*
   }
 }
 
+// ---- the common case, streamlined ------------------------------------------------
+// Every term and the reduced field on float32 columns that the staging pass
+// loads (NS of them), one event per lane of a one-wave workgroup: the
+// generic kernel above carries code for terms on other columns and types and
+// dispatches each term's operator by branches, which leaves it issue-bound
+// (about 370 VALU instructions per 64-event wave).  Here a term is
+// evaluated straight-line from uniform values -- y = x, or |x| through a
+// sign mask, then (y > c & gt) | (y < c & lt) | (y == c & eq), inverted for
+// != -- which is each IEEE comparison, NaN included (every one of y > c,
+// y < c, y == c is false for a NaN, so >, >=, <, <=, == are false and != is
+// true); the staging loads are unguarded (indices clamped into the span), and
+// the per-event walk counts in 32 bits.  The same bits as
+// event_pipeline_kernel.
+struct FastTerm {
+  int slot;          // staged column the term reads
+  uint32_t absmask;  // 0x7fffffff: |x|; 0xffffffff: x
+  float c;
+  int gt, lt, eq, inv;  // 0 or 1
+};
+
+struct FastCut {
+  const float* col[kMaxFields];  // staged columns by slot
+  FastTerm term[kMaxTerms];
+  int red;  // slot of the reduced column
+};
+
+template <int NS>
+__device__ inline float fast_pick(const float (&x)[NS], int slot) {
+  float v = x[0];
+#pragma unroll
+  for (int s = 1; s < NS; s++) v = slot == s ? x[s] : v;
+  return v;
+}
+
+template <int NS, int NT>
+__device__ inline bool fast_keep(const FastCut& cut, const float (&x)[NS]) {
+  bool k = true;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const FastTerm& ft = cut.term[t];
+    const float y = __uint_as_float(__float_as_uint(fast_pick<NS>(x, ft.slot)) & ft.absmask);
+    const bool r = ((y > ft.c) & (ft.gt != 0)) | ((y < ft.c) & (ft.lt != 0)) | ((y == ft.c) & (ft.eq != 0));
+    k = k & (r != (ft.inv != 0));
+  }
+  return k;
+}
+
+template <int NS, int NT>
+__global__ __launch_bounds__(kWave, 8) void event_pipeline_fast_kernel(const int64_t* __restrict__ offsets,
+                                                                       int64_t n, FastCut cut, int cap,
+                                                                       int64_t start, int64_t stop, int64_t step,
+                                                                       PipelineOut out) {
+  constexpr int U = 4;
+  extern __shared__ float s_stage[];
+  uint2* s_pv = (uint2*)s_stage;  // (reduced value bits, predicate flag) per particle
+  const int lane = threadIdx.x;
+  const int64_t L0 = (int64_t)blockIdx.x * kWave;
+  const int nl = (int)((n - L0) < kWave ? (n - L0) : kWave);
+  int64_t own_a = 0, own_b = 0;
+  if (lane < nl) {
+    own_a = offsets[L0 + lane];
+    own_b = offsets[L0 + lane + 1];
+  }
+  // the tile's first particle and span are wave-uniform: scalar registers
+  auto uniform = [](int64_t v) -> int64_t {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+  };
+  const int64_t E0 = uniform(__shfl(own_a, 0, kWave));
+  const int64_t span = uniform(__shfl(own_b, nl - 1, kWave)) - E0;
+  const bool staged = span <= cap;
+  if (staged && span > 0) {
+    const int sp = (int)span;
+    const float* base[NS];  // uniform: scalar base + 32-bit lane offset loads
+#pragma unroll
+    for (int s = 0; s < NS; s++) base[s] = cut.col[s] + E0;
+    for (int b = 0; b < sp; b += kWave * U) {
+      float r[NS][U];
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = b + u * kWave + lane;
+          // a 32-bit byte offset from the uniform base: one scalar-base load
+          const uint32_t off4 = (uint32_t)(q < sp ? q : sp - 1) * 4u;
+          r[s][u] = *(const float*)((const char*)base[s] + off4);
+        }
+      // every load issued before any is used (the compiler would otherwise
+      // sink each slot's loads into the branch that stores it)
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int u = 0; u < U; u++) asm volatile("" ::"v"(r[s][u]));
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = b + u * kWave + lane;
+        float x[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) x[s] = r[s][u];
+        const bool k = fast_keep<NS, NT>(cut, x);
+        if (q < sp) s_pv[q] = make_uint2(__float_as_uint(fast_pick<NS>(x, cut.red)), k ? 1u : 0u);
+      }
+    }
+  }
+  __syncthreads();
+  if (lane >= nl) return;
+  const int64_t start0 = start == kSliceNone ? 0 : start;
+  float sum = 0.0f, best_val = 0.0f, maxv = -INFINITY;
+  int best = -1, t = 0;
+  int64_t cnt = 0;
+  // The lane's event over positions [lo, hi): get(q, v, k) gives particle
+  // q's value and flag (staged: q is the 32-bit position in the stage).
+  auto walk = [&](auto lo, auto hi, auto get) {
+    using Q = decltype(lo);
+    if (step == 1 && stop == kSliceNone && start0 >= 0) {
+      // [start:]: the kept particles are the selected ones of rank >= start
+      // branch-free: each step selects (a conditional add is a select of
+      // sum + v, so the sums' bits are the serial loop's)
+      const int first = start0 < INT32_MAX ? (int)start0 : INT32_MAX;
+      int sel = 0;
+      for (Q q = lo; q < hi; q++) {
+        float v;
+        bool k;
+        get(q, v, k);
+        const bool take = k & (sel >= first);
+        const bool better = take & ((t == 0) | (v > best_val));
+        sum = take ? sum + v : sum;
+        best = better ? t : best;
+        best_val = better ? v : best_val;
+        maxv = (take & (v > maxv)) ? v : maxv;
+        t += take ? 1 : 0;
+        sel += k ? 1 : 0;
+      }
+      cnt = t;
+      return;
+    }
+    // pass 1: the number of selected particles; the slice regularised against it
+    int m = 0;
+    for (Q q = lo; q < hi; q++) {
+      float v;
+      bool k;
+      get(q, v, k);
+      m += k ? 1 : 0;
+    }
+    int64_t first;
+    cnt = slice_count(m, start, stop, step, &first);
+    // pass 2: the kept particles (selected positions first + t * step)
+    int64_t next = first, sel = 0;
+    for (Q q = lo; q < hi && t < cnt; q++) {
+      float v;
+      bool k;
+      get(q, v, k);
+      const bool take = k & (sel == next);
+      const bool better = take & ((t == 0) | (v > best_val));
+      sum = take ? sum + v : sum;
+      best = better ? t : best;
+      best_val = better ? v : best_val;
+      maxv = (take & (v > maxv)) ? v : maxv;
+      t += take ? 1 : 0;
+      next += take ? step : 0;
+      sel += k ? 1 : 0;
+    }
+  };
+  if (staged) {
+    walk((int)(own_a - E0), (int)(own_b - E0), [&](int q, float& v, bool& k) {
+      const uint2 w = s_pv[q];
+      v = __uint_as_float(w.x);
+      k = w.y != 0;
+    });
+  } else {  // a tile past the stage: the columns from memory
+    walk(own_a, own_b, [&](int64_t i, float& v, bool& k) {
+      float x[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) x[s] = cut.col[s][i];
+      k = fast_keep<NS, NT>(cut, x);
+      v = fast_pick<NS>(x, cut.red);
+    });
+  }
+  // stores from uniform bases at the tile's first event, 32-bit lane offsets
+  const uint32_t l = (uint32_t)lane;
+  if (out.num) *(int64_t*)((char*)(out.num + L0) + l * 8u) = cnt;
+  if (out.sum) *(float*)((char*)(out.sum + L0) + l * 4u) = sum;
+  if (out.argmax) *(int64_t*)((char*)(out.argmax + L0) + l * 8u) = (int64_t)best;
+  if (out.argmask) *((int8_t*)(out.argmask + L0) + l) = cnt > 0 ? 0 : 1;
+  if (out.max) *(float*)((char*)(out.max + L0) + l * 4u) = maxv;
+  if (out.maxmask) *((int8_t*)(out.maxmask + L0) + l) = cnt > 0 ? 0 : 1;
+}
+
 // ---- pairs ---------------------------------------------------------------------
 __device__ inline float pair_expr(float pt1, float eta1, float phi1, float pt2, float eta2, float phi2) {
   // 2 * pt1 * pt2 * (cosh(eta1 - eta2) - cos(phi1 - phi2)), left to right
@@ -588,6 +777,40 @@ Error awkward_hip_event_pipeline(const int64_t* offsets, int64_t n, int64_t n_pa
   const size_t lds = (size_t)st.cap * 8;  // (reduced value, predicate flag) pairs
   const unsigned grid = (unsigned)ceil_div(n, (int64_t)B * lpt);
   hipStream_t s = stream();
+  // the streamlined kernel: one event per lane, every term and the reduced
+  // field on staged float32 columns, at most four terms
+  bool fast = lpt == 1 && nterms <= 4 && st.slot[red] >= 0;
+  for (int t = 0; t < nterms; t++) fast = fast && st.slot[term_field[t]] >= 0;
+  if (fast) {
+    FastCut cut{};
+    for (int k = 0; k < nfields; k++)
+      if (st.slot[k] >= 0) cut.col[st.slot[k]] = (const float*)field_ptrs[k];
+    cut.red = st.slot[red];
+    for (int t = 0; t < nterms; t++) {
+      FastTerm& ft = cut.term[t];
+      ft.slot = st.slot[term_field[t]];
+      ft.absmask = term_abs[t] ? 0x7fffffffu : 0xffffffffu;
+      ft.c = (float)term_value[t];
+      const int op = term_op[t];
+      ft.gt = op == GT || op == GE;
+      ft.lt = op == LT || op == LE;
+      ft.eq = op == GE || op == LE || op == EQ || op == NE;
+      ft.inv = op == NE;
+    }
+    switch (st.nslots * 16 + nterms) {
+#define FCASE(NS, K) \
+  case NS * 16 + K: event_pipeline_fast_kernel<NS, K><<<grid, B, lds, s>>>(offsets, n, cut, st.cap, start, stop, step, out); break;
+#define FCASES(NS) FCASE(NS, 0) FCASE(NS, 1) FCASE(NS, 2) FCASE(NS, 3) FCASE(NS, 4)
+      FCASES(1)
+      FCASES(2)
+      FCASES(3)
+      FCASES(4)
+#undef FCASES
+#undef FCASE
+    }
+    AKHIP_LAUNCH_CHECK("event_pipeline_fast_kernel");
+    return success();
+  }
   switch (lpt * 16 + nterms) {
 #define CASE(LPT, K)                                                                                              \
   case LPT * 16 + K:                                                                                              \

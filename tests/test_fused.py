@@ -163,3 +163,49 @@ def test_combinations_runner_matches_registry_path(hip, n):
         g.replay()
         for k in range(n):
             assert same_bits(wl.content.contents[k].index.cpu().numpy(), runner.carry[k][: runner.total].cpu().numpy())
+
+
+TERM_SHAPES = [
+    [("pt", ">=", 25.0), ("eta", "abs<=", 2.4)],
+    [("pt", "<", 40.0), ("phi", "!=", 0.5), ("eta", "abs>", 0.5)],
+    [("eta", "abs<", 3.0), ("phi", "==", 0.5)],
+    [],
+    [("charge", ">", 0), ("pt", ">", 20.0)],  # an int8 column: the generic kernel
+    [("pt", ">", 10.0), ("pt", "<", 200.0), ("eta", "abs<", 2.4), ("phi", ">", -3.0), ("phi", "<", 3.0)],
+]
+
+
+@pytest.mark.parametrize("sl", [slice(1, None), slice(None, None, 2)])
+@pytest.mark.parametrize("k", range(len(TERM_SHAPES)))
+def test_event_pipeline_term_shapes(cpu, hip, k, sl):
+    """Every comparison, through abs or not, with NaN, infinities, signed
+    zeros and exact hits of the constants in the columns; float32 cuts of at
+    most four terms take the streamlined kernel, an int8 term or a fifth term
+    the generic one; both walks ([start:] in one pass, a stepped slice in
+    two) -- against the oracle backend's reference sequence."""
+    from awkward_amd.fused import EventPipeline
+
+    off, fields = events(20 + k, 20000, 3, 20)
+    rng = np.random.default_rng(k)
+    N = len(fields["pt"])
+    for name in ("pt", "eta", "phi"):
+        idx = rng.integers(0, N, N // 30)
+        fields[name][idx] = rng.choice(np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 0.5, -0.5], np.float32),
+                                       len(idx))
+    terms = TERM_SHAPES[k]
+    ev_c = ak.from_offsets(off, fields, cpu)
+    mask = None
+    for f, op, v in terms:
+        x = abs(ev_c[f]) if op.startswith("abs") else ev_c[f]
+        op = op[3:] if op.startswith("abs") else op
+        m = {">": x > v, ">=": x >= v, "<": x < v, "<=": x <= v, "==": x == v, "!=": x != v}[op]
+        mask = m if mask is None else mask & m
+    sel = (ev_c[mask] if mask is not None else ev_c)[:, sl]
+    out = EventPipeline(ak.from_offsets(off, fields, hip), terms, sl, "pt").run()
+    got = {key: v.cpu().numpy() for key, v in out.items()}
+    assert same_bits(flat(ak.num(sel))[0], got["num"])
+    assert same_bits(flat(ak.sum(sel.pt, axis=-1))[0], got["sum"])
+    want_arg = flat(ak.argmax(sel.pt, axis=-1))
+    assert same_bits(want_arg[0], got["argmax_mask"]) and same_bits(want_arg[1], got["argmax"])
+    want_max = flat(ak.max(sel.pt, axis=-1))
+    assert same_bits(want_max[0], got["max_mask"]) and same_bits(want_max[1], got["max"])
