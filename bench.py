@@ -1009,7 +1009,7 @@ def main():
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.events),
-                "kernel": "event_pipeline_kernel (awkward_hip_event_pipeline)", "kernel_ms": round(kms, 4),
+                "kernel": "event_pipeline_fast_kernel (awkward_hip_event_pipeline)", "kernel_ms": round(kms, 4),
                 "alg_bytes": alg_bytes}
 
     reductions = None if args.no_reductions else run_segmented_reductions(ak, tuned_be, torch)
