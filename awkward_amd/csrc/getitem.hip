@@ -509,7 +509,7 @@ extern "C" {
 
 // awkward_ListArray64_getitem_next_range_64 with the scan scratch in the
 // file-scope array above: no hipMalloc, hipFree or host synchronisation.
-// At most kNoAllocMaxLists (1,073,217,536) lists; calls that overlap on one
+// At most kNoAllocMaxLists (536,608,768) lists; calls that overlap on one
 // device share the array and race.
 Error awkward_hip_ListArray64_getitem_next_range_64_noalloc(int64_t* tooffsets, int64_t* tocarry,
                                                             const int64_t* fromstarts, const int64_t* fromstops,

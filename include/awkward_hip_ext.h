@@ -197,7 +197,7 @@ struct Error awkward_hip_reduce_argmin_shifts_float64(int64_t* toptr, int8_t* ma
    forbid hipMalloc).  awkward_ListArray64_getitem_next_range_64 with the
    look-back scan's scratch (tile counter and status words) in a file-scope
    1 MiB __device__ array instead of the stream's pool; no hipMalloc,
-   hipFree or host synchronisation.  At most 1,073,217,536 lists (a clear
+   hipFree or host synchronisation.  At most 536,608,768 lists (a clear
    error beyond).  Every call on one device shares that device's array, and
    each call clears what it uses first: calls that overlap on one device
    (concurrent streams, threads) race. */

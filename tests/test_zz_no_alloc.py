@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 KSLICENONE = 2**63 - 1
-MAX_LISTS = ((1 << 20) - 512) // 8 * 512 * 16  # status words x the scan tile (512 threads x 16 lists)
+MAX_LISTS = ((1 << 20) - 512) // 8 * 512 * 8  # status words x the scan tile (512 threads x 8 lists)
 
 
 def _fn(name):
