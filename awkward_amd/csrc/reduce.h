@@ -682,8 +682,10 @@ inline Error launch_reduce_v(const Op& op, const Epi& epi, const typename Op::In
 // into maxcount columns): a tile pass that queues every list longer than
 // kWaveLen in kReduceChunk-element chunks, one chunk per workgroup
 // iteration over the whole chip, and the chunk partials combined in order
-// per list.  The queue is sized from the element count, read from the
-// offsets (one host synchronisation), so no list stays with a workgroup.
+// per list.  The queue has a fixed capacity (kQueuedChunks chunks: 2^31
+// queued elements), so nothing is read back to size it and the sequence can
+// be captured; a wave whose lists do not fit keeps them with its workgroup
+// (queue_reduce), which is slower but gives the same result.
 template <typename In, class Epi>
 struct SingleSrc {
   const In* p;
@@ -701,8 +703,7 @@ __global__ __launch_bounds__(256) void segmented_reduce_queued_kernel(Op op, Epi
   reduce_tile<Op, 256, 1, 8192, Epi>(op, epi, fromptr, offsets, outlength, (int64_t)blockIdx.x * 256, sm, q, 0);
 }
 
-template <int Dummy = 0>
-__global__ void span_kernel(int64_t* out, const int64_t* offsets, int64_t n) { out[0] = offsets[n] - offsets[0]; }
+constexpr int64_t kQueuedChunks = int64_t(1) << 18;
 
 template <class Op, class Epi>
 inline Error launch_reduce_queued(const Op& op, const Epi& epi, const typename Op::In* fromptr,
@@ -710,16 +711,7 @@ inline Error launch_reduce_queued(const Op& op, const Epi& epi, const typename O
   using Part = typename Op::Part;
   using In = typename Op::In;
   if (outlength <= 0) return success();
-  int64_t elements = 0;
-  {
-    int64_t* d = (int64_t*)scratch(256);
-    if (!d) return failure("out of device memory for scratch", kSliceNone, kSliceNone, "awkward_amd/csrc/reduce.h");
-    span_kernel<><<<1, 1, 0, stream()>>>(d, offsets, outlength);
-    hipError_t err = hipMemcpyAsync(&elements, d, sizeof(int64_t), hipMemcpyDeviceToHost, stream());
-    if (err == hipSuccess) err = hipStreamSynchronize(stream());
-    if (err != hipSuccess) return hip_failure(err, "reading the element count");
-  }
-  const int64_t cap = reduce_queue_capacity(elements > 0 ? elements : 1);
+  const int64_t cap = kQueuedChunks;
   const size_t cb = ((size_t)cap * sizeof(ReduceChunk) + 255) / 256 * 256;
   const size_t pb = ((size_t)cap * sizeof(Part) + 255) / 256 * 256;
   char* m = (char*)scratch(256 + cb + pb);

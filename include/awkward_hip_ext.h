@@ -47,8 +47,8 @@ int awkward_hip_release_pools(void);
 int awkward_hip_reduce_variant_count(void);
 
 /* ---- tuned reductions: one tile variant (0 .. 9, or 10: lists longer than
-   4096 elements queued in 8192-element chunks across the chip, which reads the
-   element count from the offsets), fused epilogues ---------
+   4096 elements queued in 8192-element chunks across the chip: a fixed-capacity
+   queue, nothing read back, capturable), fused epilogues ---------
    sum:  awkward_reduce_sum_<OUT>_<IN>_64 with the variant's tile.
    max/min_masked: awkward_reduce_max/min + the empty-list mask
      (awkward_NumpyArray_reduce_mask_ByteMaskedArray_64) in one pass; mask may

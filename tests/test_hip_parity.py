@@ -560,3 +560,51 @@ def test_long_list_queue_and_shifts(t):
                                        starts=d_starts).str is None
             assert same_bits(hs[0], d_out.cpu().numpy()), (kind, variant, "shifts")
 
+
+
+def test_long_list_queue_in_a_graph():
+    """Variant 10 reads nothing back to size its queue (a fixed capacity;
+    lists that would overflow it stay with their workgroup), so a sum,
+    masked max and local argmax over very long lists can be captured:
+    every replay -- with eager library work and host synchronisations in
+    between -- equals the eager results and the oracle."""
+    import torch
+
+    from awkward_amd import dispatch
+    from awkward_amd.fused import Graph
+
+    rng = np.random.default_rng(78)
+    counts = np.array([250000, 0, 3, 9000, 100001, 1, 8192, 70000], dtype=np.int64)
+    off = offsets_from_counts(counts)
+    n = len(counts)
+    x = rng.standard_normal(int(off[-1]))
+    d_off, d_x = torch.from_numpy(off).cuda(), torch.from_numpy(x).cuda()
+    outs = {"sum": torch.empty(n, dtype=torch.float64, device="cuda"),
+            "max": torch.empty(n, dtype=torch.float64, device="cuda"),
+            "mask": torch.empty(n, dtype=torch.int8, device="cuda"),
+            "argmax": torch.empty(n, dtype=torch.int64, device="cuda")}
+
+    def step():
+        assert dispatch.reduce_sum(outs["sum"], d_x, d_off, n, dispatch.LONG_LISTS).str is None
+        assert dispatch.reduce_extremum("max", outs["max"], outs["mask"], d_x, d_off, n, -np.inf,
+                                        dispatch.LONG_LISTS).str is None
+        assert dispatch.reduce_arg("argmax", outs["argmax"], None, d_x, d_off, n, dispatch.LONG_LISTS).str is None
+        return outs
+
+    step()
+    torch.cuda.synchronize()
+    eager = {k: v.clone() for k, v in outs.items()}
+    hmax, _, _, _ = run_both("awkward_reduce_max_float64_float64_64", [np.zeros(n), x, off, n, -np.inf])
+    harg, _, _, _ = run_both("awkward_reduce_argmax_float64_64", [np.zeros(n, np.int64), x, off, off[:-1].copy(), n])
+    assert same_bits(hmax[0], eager["max"].cpu().numpy())
+    assert same_bits(np.where(harg[0] >= 0, harg[0] - off[:-1], harg[0]), eager["argmax"].cpu().numpy())
+    g = Graph(step)
+    for _ in range(3):
+        for v in outs.values():
+            v.zero_()
+        dispatch.reduce_sum(torch.empty(3, dtype=torch.float64, device="cuda"), d_x, d_off[:4], 3, 3)  # eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in outs:
+            assert torch.equal(outs[k], eager[k]), k
