@@ -58,9 +58,30 @@ def san_runtime() -> str:
     return found[-1]
 
 
+def _embed(objdir: str) -> None:
+    """jit_embed.h: the headers of the kernels compiled at run time (jit.hip),
+    as string constants -- expr.h without its <cmath> include (the run-time
+    compiler has the device math built in) and pair_jit.h."""
+    texts = {}
+    with open(os.path.join(CSRC, "expr.h")) as f:
+        texts["kJitExprH"] = f.read().replace("#include <cmath>\n", "")
+    with open(os.path.join(CSRC, "pair_jit.h")) as f:
+        texts["kJitPairH"] = f.read()
+    delim = "AKHIP_EMBED"
+    out = "// generated by awkward_amd/build.py from csrc/expr.h and csrc/pair_jit.h\n#pragma once\n"
+    for name, text in texts.items():
+        if ")" + delim in text:
+            raise ValueError(f"{name} holds the raw-string delimiter")
+        out += f'static const char {name}[] = R"{delim}({text}){delim}";\n'
+    path = os.path.join(objdir, "jit_embed.h")
+    if not os.path.exists(path) or open(path).read() != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
 def _compile(src: str, objdir: str = OBJDIR, extra=()) -> str:
     obj = os.path.join(objdir, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *extra, f"-I{objdir}", "-c", src, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{proc.stderr}")
@@ -73,11 +94,12 @@ def build(verbose: bool = True, sanitize: bool = False) -> str:
     os.makedirs(objdir, exist_ok=True)
     os.makedirs(os.path.dirname(libpath), exist_ok=True)
     sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    _embed(objdir)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
         objects = list(pool.map(lambda src: _compile(src, objdir, extra), sources))
     tmp = libpath + ".tmp"
     # the sanitized library takes its runtime from the preloaded ASan runtime
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objects, "-o", tmp]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objects, "-lhiprtc", "-o", tmp]
     if sanitize:
         cmd[3:3] = ["-Wl,--allow-shlib-undefined", "-Wl,--unresolved-symbols=ignore-all"]
     proc = subprocess.run(cmd, capture_output=True, text=True)

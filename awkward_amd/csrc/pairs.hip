@@ -18,6 +18,7 @@
 #include "combin.h"
 #include "common.h"
 #include "expr.h"
+#include "jit.h"
 
 namespace akhip {
 namespace {
@@ -202,15 +203,16 @@ __device__ inline F as_value(uint64_t w) {
 }
 
 // The program at U pairs whose elements sit at staged positions li[u] (leg 0)
-// and lj[u] (leg 1); li[u] < 0 marks an unused slot.
-template <int U, typename W>
+// and lj[u] (leg 1); li[u] < 0 marks an unused slot.  Staged columns are
+// STRIDE elements apart.
+template <int U, typename W, int STRIDE = kPRStage>
 __device__ inline void eval_staged(const ExprProgram& p, const PairSlots& ps, const W* s_col, const int (&li)[U],
                                    const int (&lj)[U], W (&out)[U]) {
   XFileV<U, W> f = 0;
 #pragma unroll
   for (int k = 0; k < kExprMaxLeaves && k < XFile<U, W>::regs; k++) {
     if (k < p.nleaves) {
-      const W* col = s_col + ps.slot[k] * kPRStage;
+      const W* col = s_col + ps.slot[k] * STRIDE;
       const bool second = p.leaf[k].leg != 0;
 #pragma unroll
       for (int u = 0; u < U; u++)
@@ -498,6 +500,17 @@ Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t 
   if (n <= 0 || (!prog && nreq == 0)) return success();
   const PairOut out{out_max, out_min, out_sum, out_argmax, out_argmin, out_mask};
   const PairSlots ps = prog ? pair_slots(p) : PairSlots{};
+  // extrema without a sum: the carries (if any), then the program's own
+  // kernel compiled at run time (jit.h), when it builds
+  const void* jit = prog && !out_sum ? pair_jit_kernel(p, ps.slot, ps.first, ps.nslots, out_max || out_argmax,
+                                                       out_min || out_argmin)
+                                     : nullptr;
+  if (jit != nullptr) {
+    if (nreq > 0) launch_pairs<float>(p, ps, 0, offsets, n, out, cr);
+    pair_jit_run(jit, p, ps.nslots, offsets, n, out_max, out_min, out_argmax, out_argmin, out_mask);
+    AKHIP_LAUNCH_CHECK("akhip_pair_jit");
+    return success();
+  }
   if (prog && p.out_type == kTFloat64) launch_pairs<double>(p, ps, prog, offsets, n, out, cr);
   else launch_pairs<float>(p, ps, prog, offsets, n, out, cr);
   AKHIP_LAUNCH_CHECK("pairs_kernel");

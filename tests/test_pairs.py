@@ -203,3 +203,145 @@ def test_combinations_outputs_alone_and_with_replacement():
         assert np.array_equal(lf.nplike.to_numpy(lf.offsets), le.nplike.to_numpy(le.offsets))
         for cf, ce in zip(lf.content.contents, le.content.contents):
             assert np.array_equal(cf.nplike.to_numpy(cf.index), ce.nplike.to_numpy(ce.index))
+
+
+def long_events(n, seed):
+    """Poisson(4) events with a few lists longer than a wave's stage (400 to
+    900 elements: their pairs are spread over the wave with the leaves read
+    from memory) and some of 40-300 elements (one staged list over the wave)."""
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(4, n)
+    counts[rng.choice(n, 4, replace=False)] = rng.integers(400, 900, 4)
+    counts[rng.choice(n, 10, replace=False)] = rng.integers(40, 300, 10)
+    counts[:4] = [0, 1, 2, 3]
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    m = int(off[-1])
+    fields = {"pt": rng.exponential(20, m).astype(np.float32), "eta": rng.normal(0, 1.5, m).astype(np.float32),
+              "phi": rng.uniform(-np.pi, np.pi, m).astype(np.float32)}
+    return off, fields
+
+
+def special_events(n, seed):
+    """Small integer pt (ties everywhere), signed zeros, NaN and +-inf, lists
+    whose first element is NaN (every pair (0, j) is NaN, the first pair
+    among them) and lists of NaN only."""
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(5, n)
+    counts[rng.choice(n, 6, replace=False)] = rng.integers(40, 120, 6)
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    m = int(off[-1])
+    pt = rng.integers(-2, 3, m).astype(np.float32)
+    pt[rng.random(m) < 0.05] = -0.0
+    pt[rng.random(m) < 0.03] = np.nan
+    pt[rng.random(m) < 0.01] = np.inf
+    pt[rng.random(m) < 0.01] = -np.inf
+    starts = off[:-1][(counts > 1) & (rng.random(n) < 0.05)]
+    pt[starts] = np.nan
+    for e in np.flatnonzero((counts > 1) & (rng.random(n) < 0.01)):
+        pt[off[e]:off[e + 1]] = np.nan
+    fields = {"pt": pt, "eta": rng.normal(0, 1.5, m).astype(np.float32),
+              "phi": rng.uniform(-np.pi, np.pi, m).astype(np.float32)}
+    return off, fields
+
+
+EXTREMA = [("max", "argmax"), ("min", "argmin"), ("argmax",), ("min",), ("max", "argmax", "min", "argmin")]
+
+
+def extrema(x, names):
+    return [getattr(x, name)() for name in names]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("names", EXTREMA, ids=["-".join(n) for n in EXTREMA])
+@pytest.mark.parametrize("make", ["poisson", "heavy", "long", "special"])
+def test_pair_extrema_equal_eager(make, names):
+    """Extrema without a sum: one wave per 64 lists, each lane reducing a run
+    of pairs and the runs' partials merged per list -- bit for bit the
+    elementwise form's ak.max / ak.argmax / ak.min / ak.argmin, NaN first
+    pairs, all-NaN lists, ties and signed zeros included."""
+    from awkward_amd.backend import HipBackend
+
+    off, fields = {"poisson": lambda: events(30000, seed=21), "heavy": lambda: heavy_events(20000, seed=22),
+                   "long": lambda: long_events(6000, seed=23), "special": lambda: special_events(20000, seed=24)}[make]()
+    ev = L.lazy(ak.from_offsets(off, fields, HipBackend()))
+    pairs = ev.combinations(2)
+    exprs = [mass_like(pairs)]
+    if make == "special":
+        exprs.append(pairs["0"].pt * pairs["1"].pt)
+    for expr in exprs:
+        outs = extrema(expr, names)
+        plan = L.Plan(outs)
+        assert plan.stats()["fused_launches"] == 1
+        for f, e in zip(plan.run(), L.compute(*outs, fuse=False)):
+            assert same(f, e)
+
+
+@pytest.mark.gpu
+def test_pair_extrema_exact_arithmetic_equal_the_oracle():
+    """Products and sums of float32 round alike everywhere: the fused extrema
+    equal the oracle's bit for bit, including the special values."""
+    from awkward_amd.backend import HipBackend
+
+    off, fields = special_events(6000, seed=25)
+    outs = {}
+    for name, be in (("hip", HipBackend()), ("oracle", oracle_backend())):
+        pairs = L.lazy(ak.from_offsets(off, fields, be)).combinations(2)
+        p0, p1 = pairs["0"], pairs["1"]
+        expr = p0.pt * p1.pt - 3 * (p0.eta + p1.eta) * p0.phi
+        plan = L.Plan(extrema(expr, ("max", "argmax", "min", "argmin")))
+        if name == "hip":
+            assert plan.stats()["fused_launches"] == 1
+        outs[name] = plan.run()
+    for g, w in zip(outs["hip"], outs["oracle"]):
+        assert same(g, w)
+
+
+@pytest.mark.gpu
+def test_pair_extrema_with_the_combinations():
+    """The extrema in the same entry point call as the n = 2 and n = 3
+    carries (the carries, then the extrema's launch)."""
+    from awkward_amd.backend import HipBackend
+
+    off, fields = long_events(5000, seed=26)
+    ev = L.lazy(ak.from_offsets(off, fields, HipBackend()))
+    pairs = ev.combinations(2)
+    expr = mass_like(pairs)
+    outs = [expr.max(), expr.argmin(), pairs, ev.combinations(3)]
+    plan = L.Plan(outs)
+    assert plan.stats()["fused_launches"] == 1
+    fused, eager = plan.run(), L.compute(*outs, fuse=False)
+    for f, e in zip(fused[:2], eager[:2]):
+        assert same(f, e)
+    for f, e in zip(fused[2:], eager[2:]):
+        lf, le = f.layout, e.layout
+        assert np.array_equal(lf.nplike.to_numpy(lf.offsets), le.nplike.to_numpy(le.offsets))
+        for cf, ce in zip(lf.content.contents, le.content.contents):
+            assert np.array_equal(cf.nplike.to_numpy(cf.index), ce.nplike.to_numpy(ce.index))
+
+
+@pytest.mark.gpu
+def test_pair_extrema_compiled_at_run_time_equal_the_interpreter(monkeypatch):
+    """Extrema without a sum run in a kernel compiled at run time for the
+    program (one more compiled kernel for a new program shape); with
+    AWKWARD_HIP_JIT=0 the interpreter runs them: the same bits."""
+    import ctypes
+
+    from awkward_amd._lib import lib
+    from awkward_amd.backend import HipBackend
+
+    compiled = lib().awkward_hip_jit_kernels
+    compiled.restype = ctypes.c_int
+    off, fields = special_events(8000, seed=27)
+    results = {}
+    for jit in ("1", "0"):
+        monkeypatch.setenv("AWKWARD_HIP_JIT", jit)
+        before = compiled()
+        pairs = L.lazy(ak.from_offsets(off, fields, HipBackend())).combinations(2)
+        p0, p1 = pairs["0"], pairs["1"]
+        expr = np.sqrt(abs(p0.pt - p1.phi)) * np.float32(1.25) + np.exp(p0.eta - p1.eta)  # a shape of its own
+        results[jit] = L.Plan(extrema(expr, ("max", "argmax", "min", "argmin"))).run()
+        assert compiled() == before + (1 if jit == "1" else 0)
+    for a, b in zip(results["1"], results["0"]):
+        assert same(a, b)
