@@ -61,14 +61,16 @@ def san_runtime() -> str:
 def _embed(objdir: str) -> None:
     """jit_embed.h: the headers of the kernels compiled at run time (jit.hip),
     as string constants -- expr.h without its <cmath> include (the run-time
-    compiler has the device math built in) and pair_jit.h."""
+    compiler has the device math built in), pair_jit.h and tuples.h."""
     texts = {}
     with open(os.path.join(CSRC, "expr.h")) as f:
         texts["kJitExprH"] = f.read().replace("#include <cmath>\n", "")
     with open(os.path.join(CSRC, "pair_jit.h")) as f:
         texts["kJitPairH"] = f.read()
+    with open(os.path.join(CSRC, "tuples.h")) as f:
+        texts["kJitTuplesH"] = f.read()
     delim = "AKHIP_EMBED"
-    out = "// generated by awkward_amd/build.py from csrc/expr.h and csrc/pair_jit.h\n#pragma once\n"
+    out = "// generated by awkward_amd/build.py from csrc/expr.h, pair_jit.h and tuples.h\n#pragma once\n"
     for name, text in texts.items():
         if ")" + delim in text:
             raise ValueError(f"{name} holds the raw-string delimiter")

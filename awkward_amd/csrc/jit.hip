@@ -2,11 +2,12 @@
 //
 // The interpreter (expr.h) runs any program, but each op costs a dispatch
 // and indexed register moves around its arithmetic: over the pairs of
-// ak.combinations that is most of the time.  Here the pair-extrema kernel
+// ak.combinations that is most of the time.  Here the pair-reduction kernel
 // (pair_jit.h) is compiled with hipRTC for the program's shape -- its ops
-// as straight-line code, each op being expr.h's own x_op_v with the op a
-// compile-time constant, so the values are the interpreter's bit for bit
-// (contraction off: no fused multiply-adds across ops) -- once per shape and
+// as straight-line code, each computed as the interpreter computes it (the
+// floating-point ones written out, the rest through expr.h's own x_op_v),
+// so the values are the interpreter's bit for bit (contraction off: no fused
+// multiply-adds across ops) -- once per shape and
 // device, and cached for the life of the process.  The build embeds expr.h
 // and pair_jit.h as text (jit_embed.h); the run-time source needs nothing
 // else from this library.
@@ -22,6 +23,7 @@
 
 #include "jit.h"
 #include "jit_embed.h"
+#include "tuples.h"
 
 namespace akhip {
 namespace {
@@ -34,16 +36,25 @@ const char kJitCommonH[] =
     "using __hip_internal::int64_t; using __hip_internal::uint64_t;\n"
     "namespace akhip { constexpr int kWave = 64; }\n";
 
-constexpr int kJitStage = 384;  // elements per staged column and wave (pjit::kStage)
+constexpr int kJitStage = 768;  // elements per staged column (pjit::kStage)
+constexpr int kJitCap = 1536;   // pair values kept per workgroup (pjit::kCap)
+constexpr int kJitLists = 128;  // lists (and threads) per workgroup (pjit::kLists)
 
-// pjit::Leaves and pjit::Out, member for member
+// pjit::Leaves, pjit::Tuples and pjit::Out, member for member
 struct JitLeaves {
   const void* data[kExprMaxLeaves];
   const int64_t* index[kExprMaxLeaves];
 };
+struct JitTuples {
+  int32_t nreq;
+  int32_t n[2];
+  const int64_t* tooffsets[2];
+  int64_t* carry[2][3];
+};
 struct JitOut {
   void* max;
   void* min;
+  void* sum;
   int64_t* argmax;
   int64_t* argmin;
   int8_t* mask;
@@ -64,10 +75,69 @@ std::string int_list(const std::vector<int>& v) {
   return s + "}";
 }
 
-// The run-time source for program p's extrema: its shape as constants and
-// its ops as straight-line code (registers R[0 .. nleaves) start with the
-// leaves, as in the interpreter).
-std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* first, int nslots, bool mx, bool mn) {
+// One op of the run-time source, over registers R0 .. R15 of canonical bits.
+// Floating-point arithmetic, comparisons, max / min and the math functions
+// are written out as the interpreter computes them (x_arith_v, x_compare,
+// x_math_v: the op in the float type, the result's bits); every other op
+// calls the interpreter's own x_op_v with the op as a constant.  (Written
+// out, the compiler keeps the math functions' rare paths -- cosf's reduction
+// of large arguments -- behind branches; through x_op_v's one-slot vectors
+// it computes them for every pair and selects.)
+std::string op_source(const ExprOp& o) {
+  const std::string d = "R" + std::to_string(o.dst), a = "R" + std::to_string(o.a);
+  const std::string b = o.bimm ? "(W)" + std::to_string(o.imm) + "ull" : "R" + std::to_string(o.b);
+  const bool f32 = o.type == kTFloat32, f64 = o.type == kTFloat64;
+  if (f32 || f64) {
+    const std::string X = f32 ? "x_f<W>(" : "x_d(", TO = f32 ? "x_from_f<W>(" : "x_from_d(";
+    const std::string A = X + a + ")", B = X + b + ")";
+    const char* bin = nullptr;
+    switch (o.code) {
+      case kXAdd: bin = "+"; break;
+      case kXSub: bin = "-"; break;
+      case kXMul: bin = "*"; break;
+      case kXDiv: bin = "/"; break;
+      default: break;
+    }
+    if (bin != nullptr) return "  " + d + " = " + TO + A + " " + bin + " " + B + ");\n";
+    const char* cmp = nullptr;
+    switch (o.code) {
+      case kXGt: cmp = ">"; break;
+      case kXGe: cmp = ">="; break;
+      case kXLt: cmp = "<"; break;
+      case kXLe: cmp = "<="; break;
+      case kXEq: cmp = "=="; break;
+      case kXNe: cmp = "!="; break;
+      default: break;
+    }
+    if (cmp != nullptr && o.to == 0) return "  " + d + " = (W)(" + A + " " + cmp + " " + B + ");\n";
+    const char* fn = nullptr;
+    switch (o.code) {
+      case kXCosh: fn = f32 ? "coshf" : "cosh"; break;
+      case kXCos: fn = f32 ? "cosf" : "cos"; break;
+      case kXSinh: fn = f32 ? "sinhf" : "sinh"; break;
+      case kXSin: fn = f32 ? "sinf" : "sin"; break;
+      case kXExp: fn = f32 ? "expf" : "exp"; break;
+      case kXSqrt: fn = f32 ? "sqrtf" : "sqrt"; break;
+      case kXAbs: fn = f32 ? "fabsf" : "fabs"; break;
+      default: break;
+    }
+    if (fn != nullptr) return "  " + d + " = " + TO + fn + "(" + A + "));\n";
+    if (o.code == kXNeg) return "  " + d + " = " + TO + "-" + A + ");\n";
+    if (o.code == kXMax || o.code == kXMin)
+      return "  " + d + " = " + TO + (o.code == kXMax ? "x_fmax(" : "x_fmin(") + A + ", " + B + "));\n";
+  }
+  return "  {\n    constexpr ExprOp o = {" + std::to_string(o.code) + ", " + std::to_string(o.type) + ", " +
+         std::to_string(o.dst) + ", " + std::to_string(o.a) + ", " + std::to_string(o.b) + ", " +
+         std::to_string(o.to) + ", " + std::to_string(o.bimm) + ", 0, " + std::to_string(o.imm) + "ull};\n" +
+         "    XV<1, W> A, B;\n    A.v = " + a + ";\n    B.v = " + b + ";\n    " + d +
+         " = x_op_v<1, W>(o, A, B).v[0];\n  }\n";
+}
+
+// The run-time source for program p's reductions: its shape as constants and
+// its ops as straight-line code (registers R0 .. R(nleaves - 1) start with
+// the leaves, as in the interpreter).
+std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* first, int nslots, bool mx, bool mn,
+                        bool sum, bool tup) {
   const int nl = p.nleaves;
   std::vector<int> vslot(nl > 0 ? nl : 1, 0), vleg(nl > 0 ? nl : 1, 0), vtype(nl > 0 ? nl : 1, 0);
   std::vector<int> vfirst(nslots > 0 ? nslots : 1, 0);
@@ -80,26 +150,20 @@ std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* 
   std::string s = "#include \"expr.h\"\nnamespace akhip {\nnamespace pjit {\n";
   s += p.wide ? "using W = uint64_t;\n" : "using W = uint32_t;\n";
   s += p.out_type == kTFloat64 ? "using F = double;\n" : "using F = float;\n";
-  s += std::string("constexpr bool kMX = ") + (mx ? "true" : "false") + ", kMN = " + (mn ? "true" : "false") + ";\n";
-  s += "constexpr int kStage = " + std::to_string(kJitStage) + ", kU = " + std::to_string(p.wide ? 2 : 4) + ";\n";
+  s += std::string("constexpr bool kMX = ") + (mx ? "true" : "false") + ", kMN = " + (mn ? "true" : "false") +
+       ", kSUM = " + (sum ? "true" : "false") + ", kTUP = " + (tup ? "true" : "false") + ";\n";
+  s += "constexpr int kStage = " + std::to_string(kJitStage) + ", kCap = " + std::to_string(kJitCap) +
+       ", kU = 1;\n";
   s += "constexpr int kNLeaves = " + std::to_string(nl) + ", kNSlots = " + std::to_string(nslots) + ";\n";
   s += "constexpr int kSlot[] = " + int_list(vslot) + ";\n";
   s += "constexpr int kLeg[] = " + int_list(vleg) + ";\n";
   s += "constexpr int kType[] = " + int_list(vtype) + ";\n";
   s += "constexpr int kFirst[] = " + int_list(vfirst) + ";\n";
   s += "__device__ inline W eval(const W (&l)[" + std::to_string(nl > 0 ? nl : 1) + "]) {\n";
-  s += "  XV<1, W> R[" + std::to_string(kExprRegs) + "] = {};\n";
-  for (int k = 0; k < nl; k++) s += "  R[" + std::to_string(k) + "].v = l[" + std::to_string(k) + "];\n";
-  for (int k = 0; k < p.nops; k++) {
-    const ExprOp& o = p.op[k];
-    s += "  {\n    constexpr ExprOp o = {" + std::to_string(o.code) + ", " + std::to_string(o.type) + ", " +
-         std::to_string(o.dst) + ", " + std::to_string(o.a) + ", " + std::to_string(o.b) + ", " +
-         std::to_string(o.to) + ", " + std::to_string(o.bimm) + ", 0, " + std::to_string(o.imm) + "ull};\n";
-    s += "    XV<1, W> B;\n";
-    s += o.bimm ? "    B.v = (W)o.imm;\n" : "    B = R[" + std::to_string(o.b) + "];\n";
-    s += "    R[" + std::to_string(o.dst) + "] = x_op_v<1, W>(o, R[" + std::to_string(o.a) + "], B);\n  }\n";
-  }
-  s += "  return R[" + std::to_string(p.out) + "].v[0];\n}\n}  // namespace pjit\n}  // namespace akhip\n";
+  for (int r = 0; r < kExprRegs; r++)
+    s += "  W R" + std::to_string(r) + " = " + (r < nl ? "l[" + std::to_string(r) + "]" : std::string("0")) + ";\n";
+  for (int k = 0; k < p.nops; k++) s += op_source(p.op[k]);
+  s += "  return R" + std::to_string(p.out) + ";\n}\n}  // namespace pjit\n}  // namespace akhip\n";
   s += "#include \"pair_jit.h\"\n";
   return s;
 }
@@ -110,10 +174,10 @@ hipFunction_t build_kernel(const std::string& src, int dev) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
   const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
-  const char* headers[] = {kJitExprH, kJitCommonH, kJitPairH};
-  const char* names[] = {"expr.h", "common.h", "pair_jit.h"};
+  const char* headers[] = {kJitExprH, kJitCommonH, kJitPairH, kJitTuplesH};
+  const char* names[] = {"expr.h", "common.h", "pair_jit.h", "tuples.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "pair_jit.hip", 3, headers, names) != HIPRTC_SUCCESS) return nullptr;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "pair_jit.hip", 4, headers, names) != HIPRTC_SUCCESS) return nullptr;
   const char* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
                         "-Xclang", "-packed-fp32-ops"};
   hipFunction_t fn = nullptr;
@@ -140,11 +204,11 @@ hipFunction_t build_kernel(const std::string& src, int dev) {
 }  // namespace
 
 const void* pair_jit_kernel(const ExprProgram& p, const int8_t* slot, const int8_t* first, int nslots, bool mx,
-                            bool mn) {
+                            bool mn, bool sum, bool tuples) {
   if (!jit_enabled()) return nullptr;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  const std::string src = pair_source(p, slot, first, nslots, mx || !mn, mn);
+  const std::string src = pair_source(p, slot, first, nslots, mx, mn, sum, tuples);
   const std::string key = std::to_string(dev) + "\n" + src;
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_kernels.find(key);
@@ -156,18 +220,32 @@ const void* pair_jit_kernel(const ExprProgram& p, const int8_t* slot, const int8
 }
 
 void pair_jit_run(const void* kernel, const ExprProgram& p, int nslots, const int64_t* offsets, int64_t n,
-                  void* out_max, void* out_min, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask) {
+                  void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin,
+                  int8_t* out_mask, int nreq, const int32_t* req_n, const int64_t* const* req_tooffsets,
+                  int64_t* const* req_carries) {
   JitLeaves lv{};
   for (int k = 0; k < p.nleaves; k++) {
     lv.data[k] = p.leaf[k].data;
     lv.index[k] = p.leaf[k].index;
   }
-  JitOut out{out_max, out_min, out_argmax, out_argmin, out_mask};
-  const unsigned grid = (unsigned)ceil_div(n, (int64_t)kWave);
-  const size_t lds = (size_t)(nslots > 0 ? nslots : 1) * kJitStage * (p.wide ? 8 : 4);
-  void* args[] = {&lv, (void*)&offsets, &n, &out};
+  JitOut out{out_max, out_min, out_sum, out_argmax, out_argmin, out_mask};
+  JitTuples tp{};
+  tp.nreq = nreq;
+  int maxn = 0, used = 0;
+  for (int k = 0; k < nreq; k++) {
+    tp.n[k] = req_n[k];
+    tp.tooffsets[k] = req_tooffsets[k];
+    for (int c = 0; c < req_n[k]; c++) tp.carry[k][c] = req_carries[used++];
+    maxn = req_n[k] > maxn ? req_n[k] : maxn;
+  }
+  const unsigned grid = (unsigned)ceil_div(n, (int64_t)kJitLists);
+  size_t lds = (size_t)(nslots > 0 ? nslots : 1) * kJitStage * (p.wide ? 8 : 4) +
+               (size_t)kJitCap * (p.out_type == kTFloat64 ? 8 : 4);
+  const size_t lds_tuples = (size_t)maxn * kTupPad * sizeof(int);  // the carries' stage reuses the same LDS
+  if (lds < lds_tuples) lds = lds_tuples;
+  void* args[] = {&lv, (void*)&offsets, &n, &out, &tp};
   // a failed launch is reported by the caller's launch check
-  (void)hipModuleLaunchKernel((hipFunction_t)kernel, grid, 1, 1, kWave, 1, 1, (unsigned)lds, stream(), args,
+  (void)hipModuleLaunchKernel((hipFunction_t)kernel, grid, 1, 1, kJitLists, 1, 1, (unsigned)lds, stream(), args,
                               nullptr);
 }
 

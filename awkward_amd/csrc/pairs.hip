@@ -19,13 +19,14 @@
 #include "common.h"
 #include "expr.h"
 #include "jit.h"
+#include "tuples.h"
 
 namespace akhip {
 namespace {
 
 const char* kFile = "awkward_amd/csrc/pairs.hip";
 
-constexpr int kPRLists = 128;  // lists (and threads) per workgroup
+constexpr int kPRLists = kTupLists;  // lists (and threads) per workgroup
 constexpr int kPRCap = 2048;   // pair values kept per workgroup
 constexpr int kPRStage = 768;  // elements per staged column
 
@@ -56,11 +57,6 @@ struct CombReq {
   int64_t* carry[2][3];
 };
 
-constexpr int kTupPer = 8;                     // consecutive slots a thread decodes per chunk
-constexpr int kTupChunk = kPRLists * kTupPer;  // slots per chunk
-constexpr int kTupPad = kTupChunk + kTupChunk / kTupPer;
-__device__ inline int tup_pad(int i) { return i + (i >> 3); }  // thread t's 8 slots at t * 9: no bank conflicts
-
 // The rank-r N-tuple (lexicographic) of a list of m items, in 32 bits.
 template <int N>
 __device__ inline void decode_tuple32(int r, int m, int (&x)[N]) {
@@ -78,100 +74,6 @@ __device__ inline void decode_tuple32(int r, int m, int (&x)[N]) {
     else unrank3(r, m, x64);
 #pragma unroll
     for (int q = 0; q < N; q++) x[q] = (int)x64[q];
-  }
-}
-
-// The next tuple of the list in lexicographic order; false past its last.
-template <int N>
-__device__ inline bool next_tuple(int (&x)[N], int m) {
-  if constexpr (N == 2) {
-    if (++x[1] == m) {
-      ++x[0];
-      x[1] = x[0] + 1;
-    }
-    return x[0] < m - 1;
-  } else {
-    if (++x[2] == m) {
-      if (++x[1] == m - 1) {
-        ++x[0];
-        x[1] = x[0] + 1;
-      }
-      x[2] = x[1] + 1;
-    }
-    return x[0] < m - 2;
-  }
-}
-
-// Writes the N-tuples of this workgroup's lists, in chunks of kTupChunk
-// slots: thread t decodes slots [t * 8, t * 8 + 8) of the chunk -- one
-// binary search over the group's tuple offsets and one table decode for the
-// first, then steps in lexicographic order -- into an LDS stage as element
-// positions relative to the group's first element, and the chunk is then
-// stored with each carry written 64 consecutive slots per wave instruction.
-// Group-relative values are 32-bit (the caller checks that the group's
-// elements and tuples fit).  s_tbl / s_offl: the group's tuple and element
-// offsets relative to its first; stage: N * kTupPad ints.
-template <int N>
-__device__ inline void group_tuples(int64_t* const* carry, int64_t g0, int64_t E0, const int* s_tbl,
-                                    const int* s_offl, int nl, int* stage) {
-  const int tid = threadIdx.x;
-  const int T = s_tbl[nl];
-  for (int c0 = 0; c0 < T; c0 += kTupChunk) {
-    const int cn = T - c0 < kTupChunk ? T - c0 : kTupChunk;
-    int q = c0 + tid * kTupPer;
-    const int qe = q + kTupPer < c0 + cn ? q + kTupPer : c0 + cn;
-    if (q < qe) {
-      int lo = 0, hi = nl;  // the last list e with s_tbl[e] <= q
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_tbl[mid] <= q) lo = mid;
-        else hi = mid;
-      }
-      int e = lo;
-      int base = s_offl[e], m = s_offl[e + 1] - base;
-      int x[N];
-      decode_tuple32<N>(q - s_tbl[e], m, x);
-      for (;;) {
-        const int at = tup_pad(q - c0);
-#pragma unroll
-        for (int c = 0; c < N; c++) stage[c * kTupPad + at] = base + x[c];
-        if (++q >= qe) break;
-        if (!next_tuple<N>(x, m)) {  // on to the next list with tuples
-          do {
-            e++;
-          } while (s_tbl[e + 1] == s_tbl[e]);
-          base = s_offl[e];
-          m = s_offl[e + 1] - base;
-#pragma unroll
-          for (int c = 0; c < N; c++) x[c] = c;
-        }
-      }
-    }
-    __syncthreads();
-    // two consecutive slots per lane, one 16-byte store per carry: the
-    // pairs start at an even absolute slot (an odd first slot goes alone)
-    const int64_t gbase = g0 + c0;
-    const int lead = (int)(gbase & 1);
-    if (lead && tid == 0) {
-#pragma unroll
-      for (int c = 0; c < N; c++) carry[c][gbase] = E0 + stage[c * kTupPad + tup_pad(0)];
-    }
-    for (int i = lead + 2 * tid; i < cn; i += 2 * kPRLists) {
-      const int a0 = tup_pad(i), a1 = tup_pad(i + 1);
-#pragma unroll
-      for (int c = 0; c < N; c++) {
-        int64_t* dst = carry[c] + gbase;
-        const int64_t v0 = E0 + stage[c * kTupPad + a0];
-        if (i + 1 < cn) {
-          typedef long long v2 __attribute__((ext_vector_type(2)));
-          const v2 v = {(long long)v0, (long long)(E0 + stage[c * kTupPad + a1])};
-          *(v2*)(dst + i) = v;
-        } else {
-          dst[i] = v0;
-        }
-      }
-    }
-    __syncthreads();  // the stage is rewritten by the next chunk
   }
 }
 
@@ -257,8 +159,8 @@ __global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlot
         s_tbl[t] = (int)(s_tb[t] - g0);
       }
       __syncthreads();
-      if (cr.n[k] == 2) group_tuples<2>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage);
-      else group_tuples<3>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage);
+      if (cr.n[k] == 2) group_tuples<2>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, decode_tuple32<2>);
+      else group_tuples<3>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, decode_tuple32<3>);
     } else {
       if (cr.n[k] == 2) group_tuples_wide<2>(cr.carry[k], s_tb, s_off, nl);
       else group_tuples_wide<3>(cr.carry[k], s_tb, s_off, nl);
@@ -500,14 +402,14 @@ Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t 
   if (n <= 0 || (!prog && nreq == 0)) return success();
   const PairOut out{out_max, out_min, out_sum, out_argmax, out_argmin, out_mask};
   const PairSlots ps = prog ? pair_slots(p) : PairSlots{};
-  // extrema without a sum: the carries (if any), then the program's own
-  // kernel compiled at run time (jit.h), when it builds
-  const void* jit = prog && !out_sum ? pair_jit_kernel(p, ps.slot, ps.first, ps.nslots, out_max || out_argmax,
-                                                       out_min || out_argmin)
-                                     : nullptr;
+  // the program's own kernel compiled at run time (jit.h), with the
+  // carries, when it builds
+  const void* jit = prog ? pair_jit_kernel(p, ps.slot, ps.first, ps.nslots, out_max || out_argmax,
+                                           out_min || out_argmin, out_sum != nullptr, nreq > 0)
+                         : nullptr;
   if (jit != nullptr) {
-    if (nreq > 0) launch_pairs<float>(p, ps, 0, offsets, n, out, cr);
-    pair_jit_run(jit, p, ps.nslots, offsets, n, out_max, out_min, out_argmax, out_argmin, out_mask);
+    pair_jit_run(jit, p, ps.nslots, offsets, n, out_max, out_min, out_sum, out_argmax, out_argmin, out_mask, nreq,
+                 req_n, req_tooffsets, req_carries);
     AKHIP_LAUNCH_CHECK("akhip_pair_jit");
     return success();
   }
