@@ -115,6 +115,19 @@ extern "C" __global__ __launch_bounds__(128) void akhip_pair_jit(akhip::pjit::Le
   const int64_t L0 = (int64_t)blockIdx.x * kLists;
   const int nl = (int)((n - L0) < kLists ? (n - L0) : kLists);
   for (int t = tid; t <= nl; t += kLists) s_off[t] = offsets[L0 + t];
+  // the carries' tuple offsets, read before any store: a load issued after
+  // stores waits for them to complete (loads and stores share one counter)
+  int64_t t_own[2] = {0, 0}, t_first[2] = {0, 0}, t_last[2] = {0, 0};
+  if constexpr (kTUP) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (k < tp.nreq) {
+        if (tid < nl) t_own[k] = tp.tooffsets[k][L0 + tid];
+        t_first[k] = tp.tooffsets[k][L0];
+        t_last[k] = tp.tooffsets[k][L0 + nl];
+      }
+    }
+  }
   __syncthreads();
   const int64_t E0 = s_off[0];
   const int64_t span = s_off[nl] - E0;
@@ -268,23 +281,28 @@ extern "C" __global__ __launch_bounds__(128) void akhip_pair_jit(akhip::pjit::Le
     int* stage = (int*)s_dyn;
     int* s_tbl = s_pbeg;
     __shared__ int s_offl[kLists + 1];
-    for (int k = 0; k < tp.nreq; k++) {
-      const int64_t* to = tp.tooffsets[k];
-      const int64_t g0 = to[L0];
-      const bool narrow = span >= 0 && span < (int64_t)0x7fffffff && to[L0 + nl] - g0 < (int64_t)0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (k >= tp.nreq) break;
+      const int64_t g0 = t_first[k];
+      const bool narrow = span >= 0 && span < (int64_t)0x7fffffff && t_last[k] - g0 < (int64_t)0x7fffffff;
       __syncthreads();  // the previous request's stage and s_tbl
       if (narrow) {  // uniform
-        for (int t = tid; t <= nl; t += kLists) {
-          s_offl[t] = (int)(s_off[t] - E0);
-          s_tbl[t] = (int)(to[L0 + t] - g0);
+        if (tid < nl) {
+          s_offl[tid] = (int)(s_off[tid] - E0);
+          s_tbl[tid] = (int)(t_own[k] - g0);
+        }
+        if (tid == 0) {
+          s_offl[nl] = (int)(s_off[nl] - E0);
+          s_tbl[nl] = (int)(t_last[k] - g0);
         }
         __syncthreads();
         if (tp.n[k] == 2) group_tuples<2>(tp.carry[k], g0, E0, s_tbl, s_offl, nl, stage, tuple_by_count<2>);
         else group_tuples<3>(tp.carry[k], g0, E0, s_tbl, s_offl, nl, stage, tuple_by_count<3>);
       } else if (tid < nl) {
         const int64_t a = s_off[tid], m = s_off[tid + 1] - a;
-        if (tp.n[k] == 2) list_tuples<2>(tp.carry[k], to[L0 + tid], a, m);
-        else list_tuples<3>(tp.carry[k], to[L0 + tid], a, m);
+        if (tp.n[k] == 2) list_tuples<2>(tp.carry[k], t_own[k], a, m);
+        else list_tuples<3>(tp.carry[k], t_own[k], a, m);
       }
     }
   }

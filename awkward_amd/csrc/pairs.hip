@@ -145,11 +145,24 @@ __global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlot
   const int64_t L0 = (int64_t)blockIdx.x * kPRLists;
   const int nl = (int)((n - L0) < kPRLists ? (n - L0) : kPRLists);
   for (int t = tid; t <= nl; t += kPRLists) s_off[t] = offsets[L0 + t];
+  // the requests' tuple offsets (entries tid and tid + 128), read before any
+  // store: a load issued after stores waits for them to complete (loads and
+  // stores share one counter)
+  int64_t t_pre[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+      if (k < cr.nreq && tid + j * kPRLists <= nl) t_pre[k][j] = cr.tooffsets[k][L0 + tid + j * kPRLists];
   // phase A: the carries of the requested combinations (write-bound; the
   // program's evaluation below is ALU-bound, and the workgroups on a CU
   // overlap the two)
-  for (int k = 0; k < cr.nreq; k++) {
-    for (int t = tid; t <= nl; t += kPRLists) s_tb[t] = cr.tooffsets[k][L0 + t];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (k >= cr.nreq) break;
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+      if (tid + j * kPRLists <= nl) s_tb[tid + j * kPRLists] = t_pre[k][j];
     __syncthreads();
     const int64_t E0 = s_off[0], g0 = s_tb[0];
     const bool narrow = s_off[nl] - E0 < (int64_t)INT32_MAX && s_tb[nl] - g0 < (int64_t)INT32_MAX;  // uniform
@@ -159,8 +172,8 @@ __global__ __launch_bounds__(kPRLists) void pairs_kernel(ExprProgram p, PairSlot
         s_tbl[t] = (int)(s_tb[t] - g0);
       }
       __syncthreads();
-      if (cr.n[k] == 2) group_tuples<2>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, decode_tuple32<2>);
-      else group_tuples<3>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, decode_tuple32<3>);
+      if (cr.n[k] == 2) group_tuples<2>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, tuple_by_count<2>);
+      else group_tuples<3>(cr.carry[k], g0, E0, s_tbl, s_offl, nl, stage, tuple_by_count<3>);
     } else {
       if (cr.n[k] == 2) group_tuples_wide<2>(cr.carry[k], s_tb, s_off, nl);
       else group_tuples_wide<3>(cr.carry[k], s_tb, s_off, nl);
