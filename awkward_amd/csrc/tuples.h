@@ -12,7 +12,8 @@ constexpr int kTupLists = 128;                  // lists (and threads) per workg
 constexpr int kTupPer = 8;                      // consecutive slots a thread decodes per chunk
 constexpr int kTupChunk = kTupLists * kTupPer;  // slots per chunk
 constexpr int kTupPad = kTupChunk + kTupChunk / kTupPer;
-__device__ inline int tup_pad(int i) { return i + (i >> 3); }  // thread t's 8 slots at t * 9: no bank conflicts
+// thread t's kTupPer slots at t * (kTupPer + 1): an odd stride, no bank conflicts
+__device__ inline int tup_pad(int i) { return i + i / kTupPer; }
 
 // The next tuple of the list in lexicographic order; false past its last.
 template <int N>
@@ -56,7 +57,7 @@ __device__ inline void tuple_by_count(int r, int m, int (&x)[N]) {
 }
 
 // Writes the N-tuples of this workgroup's lists, in chunks of kTupChunk
-// slots: thread t decodes slots [t * 8, t * 8 + 8) of the chunk -- one
+// slots: thread t decodes slots [t * kTupPer, (t + 1) * kTupPer) of the chunk -- one
 // binary search over the group's tuple offsets and one decode (first(r, m,
 // x): the rank-r tuple of a list of m items) for the first, then steps in
 // lexicographic order -- into an LDS stage as element positions relative to
