@@ -108,9 +108,26 @@ def build(verbose: bool = True, sanitize: bool = False) -> str:
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
     os.replace(tmp, libpath)
+    if not sanitize:
+        jit_selftest(libpath)
     if verbose:
         print(f"built {libpath} from {len(sources)} sources for {ARCH}")
     return libpath
+
+
+def jit_selftest(libpath: str = LIBPATH) -> None:
+    """Compiles the run-time pair kernel of representative programs for ARCH
+    through the built library's hipRTC path (no device needed), so that a
+    template that no longer compiles fails the build rather than sending
+    every call to the interpreter."""
+    import ctypes
+
+    lib = ctypes.CDLL(libpath)
+    lib.awkward_hip_jit_selftest.argtypes = [ctypes.c_char_p]
+    lib.awkward_hip_jit_selftest.restype = ctypes.c_int
+    failed = lib.awkward_hip_jit_selftest(ARCH.encode())
+    if failed:
+        raise RuntimeError(f"{failed} run-time pair kernel(s) failed to compile for {ARCH} (log above)")
 
 
 if __name__ == "__main__":

@@ -168,27 +168,25 @@ std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* 
   return s;
 }
 
-// Compiles and loads the source on device dev; null (after one message on
-// stderr) when hipRTC cannot build it.
-hipFunction_t build_kernel(const std::string& src, int dev) {
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
-  const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+// Compiles the source for a target (`--offload-arch` value) into a code
+// object; false, with one message and the compiler's log on stderr, when
+// hipRTC cannot build it.
+bool compile_code(const std::string& src, const std::string& target, std::vector<char>& code) {
+  const std::string arch = "--offload-arch=" + target;
   const char* headers[] = {kJitExprH, kJitCommonH, kJitPairH, kJitTuplesH};
   const char* names[] = {"expr.h", "common.h", "pair_jit.h", "tuples.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "pair_jit.hip", 4, headers, names) != HIPRTC_SUCCESS) return nullptr;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "pair_jit.hip", 4, headers, names) != HIPRTC_SUCCESS) return false;
   const char* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
                         "-Xclang", "-packed-fp32-ops"};
-  hipFunction_t fn = nullptr;
-  if (hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts) == HIPRTC_SUCCESS) {
+  bool ok = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts) == HIPRTC_SUCCESS;
+  if (ok) {
     size_t size = 0;
-    hiprtcGetCodeSize(prog, &size);
-    std::vector<char> code(size);
-    hipModule_t mod;
-    if (size > 0 && hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS &&
-        hipModuleLoadData(&mod, code.data()) == hipSuccess)
-      if (hipModuleGetFunction(&fn, mod, "akhip_pair_jit") != hipSuccess) fn = nullptr;
+    ok = hiprtcGetCodeSize(prog, &size) == HIPRTC_SUCCESS && size > 0;
+    if (ok) {
+      code.resize(size);
+      ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+    }
   } else {
     size_t size = 0;
     hiprtcGetProgramLogSize(prog, &size);
@@ -198,6 +196,20 @@ hipFunction_t build_kernel(const std::string& src, int dev) {
                  log.c_str());
   }
   hiprtcDestroyProgram(&prog);
+  return ok;
+}
+
+// Compiles and loads the source on device dev; null when it cannot be built
+// or loaded.
+hipFunction_t build_kernel(const std::string& src, int dev) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
+  std::vector<char> code;
+  if (!compile_code(src, prop.gcnArchName, code)) return nullptr;
+  hipModule_t mod;
+  hipFunction_t fn = nullptr;
+  if (hipModuleLoadData(&mod, code.data()) != hipSuccess) return nullptr;
+  if (hipModuleGetFunction(&fn, mod, "akhip_pair_jit") != hipSuccess) return nullptr;
   return fn;
 }
 
@@ -258,6 +270,55 @@ extern "C" {
 int awkward_hip_jit_kernels(void) {
   std::lock_guard<std::mutex> lock(akhip::g_mu);
   return akhip::g_built;
+}
+
+// Compiles, without a device, the pair kernels of two representative
+// programs for `target` (e.g. "gfx950"): 2 * pt0 * pt1 * (cosh(eta0 - eta1) -
+// cos(phi0 - phi1)) in float32 with every reduction and the n = 2 / 3
+// carries, and the same in float64 after an int8 leaf's cast (an op the
+// generated source takes through x_op_v).  0 when both build, else the
+// number that failed (the compiler's log on stderr).
+int awkward_hip_jit_selftest(const char* target) {
+  using namespace akhip;
+  int failed = 0;
+  for (int wide = 0; wide < 2; wide++) {
+    ExprProgram p{};
+    const int ft = wide ? kTFloat64 : kTFloat32;
+    p.nleaves = 6 + wide;
+    p.out_type = ft;
+    p.wide = wide;
+    for (int k = 0; k < 6; k++) {
+      p.leaf[k].type = kTFloat32;
+      p.leaf[k].leg = k & 1;
+    }
+    int8_t slot[7] = {0, 0, 1, 1, 2, 2, 3}, first[4] = {0, 2, 4, 6};
+    int nops = 0;
+    auto op = [&](int code, int type, int dst, int a, int b, int to = 0, int bimm = 0, uint64_t imm = 0) {
+      p.op[nops++] = ExprOp{(uint8_t)code, (uint8_t)type, (uint8_t)dst, (uint8_t)a, (uint8_t)b, (uint8_t)to,
+                            (uint8_t)bimm, 0, imm};
+    };
+    op(kXMul, kTFloat32, 8, 0, 0, 0, 1, 0x40000000u);
+    op(kXMul, kTFloat32, 8, 8, 1);
+    op(kXSub, kTFloat32, 9, 2, 3);
+    op(kXCosh, kTFloat32, 9, 9, 9);
+    op(kXSub, kTFloat32, 10, 4, 5);
+    op(kXCos, kTFloat32, 10, 10, 10);
+    op(kXSub, kTFloat32, 9, 9, 10);
+    op(kXMul, kTFloat32, 8, 8, 9);
+    if (wide) {
+      p.leaf[6].type = kTInt8;
+      op(kXCast, kTFloat32, 8, 8, 8, kTFloat64);
+      op(kXCast, kTInt8, 11, 6, 6, kTFloat64);
+      op(kXAdd, kTFloat64, 8, 8, 11);
+    }
+    p.nops = nops;
+    p.nregs = 12;
+    p.out = 8;
+    const std::string src = pair_source(p, slot, first, 3 + wide, true, true, true, true);
+    std::vector<char> code;
+    if (!compile_code(src, target, code)) failed++;
+  }
+  return failed;
 }
 
 }  // extern "C"

@@ -305,6 +305,12 @@ struct Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, i
 /* jit_kernels: host only -- the number of kernels compiled at run time so
      far in this process (one per program shape and device). */
 int awkward_hip_jit_kernels(void);
+/* jit_selftest: host only, no device needed -- compiles the run-time pair
+     kernel of two representative programs (float32 with every reduction and
+     the n = 2 / 3 carries; float64 with an int8 leaf's cast) for target (an
+     --offload-arch value such as "gfx950"): 0 when both build, else the
+     number that failed (the compiler's log on stderr).  The build runs it. */
+int awkward_hip_jit_selftest(const char* target);
 
 /* ---- batch runner: many arrays of one dtype, persistent work queues ------------
    items: device table of 64-byte rows {data, offsets, out_sum, out_min,

@@ -57,3 +57,16 @@ def test_product_does_not_import_oracle():
                     text = f.read()
                 assert "import oracle" not in text and "from oracle" not in text, fn
                 assert "liboracle" not in text, fn
+
+
+def test_run_time_pair_kernels_compile():
+    """The pair kernels the library compiles at run time (hipRTC) build for
+    gfx950 from the embedded sources, without a device."""
+    import ctypes
+
+    from awkward_amd._lib import LIBPATH
+
+    lib = ctypes.CDLL(LIBPATH)
+    lib.awkward_hip_jit_selftest.argtypes = [ctypes.c_char_p]
+    lib.awkward_hip_jit_selftest.restype = ctypes.c_int
+    assert lib.awkward_hip_jit_selftest(b"gfx950") == 0
