@@ -7,10 +7,10 @@
 // as straight-line code, each computed as the interpreter computes it (the
 // floating-point ones written out, the rest through expr.h's own x_op_v),
 // so the values are the interpreter's bit for bit (contraction off: no fused
-// multiply-adds across ops) -- once per shape and
-// device, and cached for the life of the process.  The build embeds expr.h
-// and pair_jit.h as text (jit_embed.h); the run-time source needs nothing
-// else from this library.
+// multiply-adds across ops) -- once per shape and device, and cached for the
+// life of the process (a handful of shapes per program; the modules stay
+// loaded).  The build embeds expr.h, pair_jit.h and tuples.h as text
+// (jit_embed.h); the run-time source needs nothing else from this library.
 
 #include <hip/hiprtc.h>
 
@@ -38,7 +38,7 @@ const char kJitCommonH[] =
 
 constexpr int kJitStage = 768;  // elements per staged column (pjit::kStage)
 constexpr int kJitCap = 1536;   // pair values kept per workgroup (pjit::kCap)
-constexpr int kJitLists = 128;  // lists (and threads) per workgroup (pjit::kLists)
+constexpr int kJitLists = kTupLists;  // lists (and threads) per workgroup (pjit::kLists)
 
 // pjit::Leaves, pjit::Tuples and pjit::Out, member for member
 struct JitLeaves {
