@@ -345,3 +345,33 @@ def test_pair_extrema_compiled_at_run_time_equal_the_interpreter(monkeypatch):
         assert compiled() == before + (1 if jit == "1" else 0)
     for a, b in zip(results["1"], results["0"]):
         assert same(a, b)
+
+
+@pytest.mark.gpu
+def test_pair_program_with_integer_legs(monkeypatch):
+    """int8 legs multiplied in int8 (wrapping), cast and mixed with float32
+    -- the ops the compiled kernel takes through the interpreter's own op
+    code next to ones it writes out: compiled, interpreted and the oracle
+    agree bit for bit (exact IEEE arithmetic)."""
+    from awkward_amd.backend import HipBackend
+
+    rng = np.random.default_rng(28)
+    counts = rng.poisson(5, 12000)
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    m = int(off[-1])
+    fields = {"pt": rng.integers(-50, 50, m).astype(np.float32),
+              "charge": rng.integers(-128, 128, m).astype(np.int8)}
+    results = {}
+    for name, be, jit in (("jit", HipBackend(), "1"), ("interpreted", HipBackend(), "0"),
+                          ("oracle", oracle_backend(), "1")):
+        monkeypatch.setenv("AWKWARD_HIP_JIT", jit)
+        pairs = L.lazy(ak.from_offsets(off, fields, be)).combinations(2)
+        p0, p1 = pairs["0"], pairs["1"]
+        expr = (p0.charge * p1.charge) * p0.pt - p1.pt
+        plan = L.Plan(reductions(expr))
+        if name != "oracle":
+            assert plan.stats()["fused_launches"] == 1
+        results[name] = plan.run()
+    for a, b, c in zip(results["jit"], results["interpreted"], results["oracle"]):
+        assert same(a, b) and same(a, c)
