@@ -140,11 +140,13 @@ std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* 
                         bool sum, bool tup) {
   const int nl = p.nleaves;
   std::vector<int> vslot(nl > 0 ? nl : 1, 0), vleg(nl > 0 ? nl : 1, 0), vtype(nl > 0 ? nl : 1, 0);
+  std::vector<int> vindexed(nl > 0 ? nl : 1, 0);
   std::vector<int> vfirst(nslots > 0 ? nslots : 1, 0);
   for (int k = 0; k < nl; k++) {
     vslot[k] = slot[k];
     vleg[k] = p.leaf[k].leg != 0;
     vtype[k] = p.leaf[k].type;
+    vindexed[k] = p.leaf[k].index != nullptr;
   }
   for (int s = 0; s < nslots; s++) vfirst[s] = first[s];
   std::string s = "#include \"expr.h\"\nnamespace akhip {\nnamespace pjit {\n";
@@ -158,6 +160,7 @@ std::string pair_source(const ExprProgram& p, const int8_t* slot, const int8_t* 
   s += "constexpr int kSlot[] = " + int_list(vslot) + ";\n";
   s += "constexpr int kLeg[] = " + int_list(vleg) + ";\n";
   s += "constexpr int kType[] = " + int_list(vtype) + ";\n";
+  s += "constexpr bool kIndexed[] = " + int_list(vindexed) + ";\n";
   s += "constexpr int kFirst[] = " + int_list(vfirst) + ";\n";
   s += "__device__ inline W eval(const W (&l)[" + std::to_string(nl > 0 ? nl : 1) + "]) {\n";
   for (int r = 0; r < kExprRegs; r++)

@@ -6,9 +6,10 @@
 //
 // The including source defines, in akhip::pjit, the program's shape as
 // constants -- W (register word), F (output float), kMX / kMN / kSUM (which
-// reductions), kTUP (carries written), kStage (elements per staged column), kCap (pair values kept
-// per workgroup), kU (pairs a thread evaluates together), kNLeaves, kNSlots,
-// kSlot[] / kLeg[] / kType[] per leaf, kFirst[] per staged column -- and
+// reductions), kTUP (carries written), kStage (elements per staged column),
+// kCap (pair values kept per workgroup), kU (pairs a thread evaluates
+// together), kNLeaves, kNSlots, kSlot[] / kLeg[] / kType[] / kIndexed[] per
+// leaf, kFirst[] per staged column -- and
 // `eval(l)`, the program as straight-line code over the leaves' canonical
 // values, each op computed as expr.h's interpreter computes it, so every
 // value is the interpreter's, bit for bit (the source is compiled with
@@ -79,7 +80,12 @@ struct Out {
   int8_t* mask;  // 1 for a list without pairs
 };
 
-__device__ inline ExprLeaf leaf_of(const Leaves& lv, int k) { return ExprLeaf{lv.data[k], lv.index[k], kType[k], 0}; }
+// leaf k as the interpreter's loader takes it; whether it reads through an
+// index is part of the program's shape (kIndexed), so for the plain columns
+// the loader's index test folds away
+__device__ inline ExprLeaf leaf_of(const Leaves& lv, int k) {
+  return ExprLeaf{lv.data[k], kIndexed[k] ? lv.index[k] : nullptr, kType[k], 0};
+}
 
 __device__ inline F as_f(W w) {
   if constexpr (sizeof(F) == 4) return __uint_as_float((uint32_t)w);
