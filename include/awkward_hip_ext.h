@@ -286,10 +286,10 @@ struct Error awkward_hip_pair_expr(float* out, const float* pt1, const float* et
      from -inf / +inf, argmax / argmin are local pair indices (first of
      equals, -1 without pairs), sums accumulate left to right in the
      program's type, mask is 1 for a list without pairs; any output may be
-     NULL.  Without a sum, the extrema run in a kernel compiled at run time
-     (hipRTC) for the program's shape on first use -- the program's ops as
-     straight-line code, the same values as the interpreter's -- and cached
-     per device; AWKWARD_HIP_JIT=0 in the environment keeps the interpreter. */
+     NULL.  The reductions run in a kernel compiled at run time (hipRTC) for
+     the program's shape on first use -- the program's ops as straight-line
+     code, the same values as the interpreter's -- and cached per device;
+     AWKWARD_HIP_JIT=0 in the environment keeps the interpreter. */
 struct Error awkward_hip_pair_reduce(const int64_t* offsets, int64_t n, const void* program, void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
 /* combinations_pairs: ak.combinations(array, k) for up to two k in {2, 3}
      (without replacement) -- the reference's awkward_ListArray_combinations
@@ -298,9 +298,9 @@ struct Error awkward_hip_pair_reduce(const int64_t* offsets, int64_t n, const vo
      tuple offsets that awkward_ListArray64_combinations_length_64 wrote for
      these offsets) and req_carries (req_n[0] carry pointers, then req_n[1])
      are HOST arrays of device pointers; every carry holds the request's
-     tuple count; carries are absolute element positions.  With a run-time
-     compiled extrema kernel (see pair_reduce) this is two launches: the
-     carries, then the extrema. */
+     tuple count; carries are absolute element positions.  With a program,
+     the run-time compiled kernel of pair_reduce also writes the carries
+     (one launch). */
 struct Error awkward_hip_combinations_pairs(const int64_t* offsets, int64_t n, int32_t nreq, const int32_t* req_n, const int64_t* const* req_tooffsets, int64_t* const* req_carries, const void* program, void* out_max, void* out_min, void* out_sum, int64_t* out_argmax, int64_t* out_argmin, int8_t* out_mask);
 /* jit_kernels: host only -- the number of kernels compiled at run time so
      far in this process (one per program shape and device). */
