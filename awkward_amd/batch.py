@@ -20,8 +20,9 @@ are made on the device, run and dropped before the next wave.
 ``variants`` selects the reduce tile per dtype (the ten tiles of
 ``dispatch.VARIANTS``); the autotuner (``tune()``) times them on the batch
 itself and keeps the fastest.  ``variants="learned"`` instead enqueues, per
-dtype group, the dispatcher's statistics pass over all the group's offsets,
-the MLP inference and the ten tile launches predicated on its choice
+dtype group, the dispatcher's launch (a statistics pass over all the group's
+offsets whose last workgroup runs the MLP inference) and the ten tile
+launches predicated on its choice
 (``awkward_hip_batch_reduce_<T>_chosen``) -- still no host synchronisation.
 """
 

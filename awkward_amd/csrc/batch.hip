@@ -278,7 +278,7 @@ Error awkward_hip_batch_reset(unsigned long long* counter) {
 
 // tables / total_tiles (host arrays of 10): the item table and tile count
 // built for each variant's lists per tile; choice (device): the variant to
-// run (awkward_hip_dispatch_infer's output).  All ten launches are enqueued,
+// run (awkward_hip_dispatch_choose's or _infer's output).  All ten launches are enqueued,
 // each leaves at once unless chosen, so no host synchronisation is needed.
 // total_elements: the batch's element count (sizes the long-list queue).
 #define BATCH_REDUCE(TN, T, S)                                                                                   \

@@ -15,7 +15,10 @@
 //   awkward_hip_dispatch_infer            features -> MLP -> choice, one
 //       workgroup per slot; a feature outside the training range selects the
 //       caller's fallback (the autotuner table's entry) instead;
-//   awkward_hip_mlp_forward               the same network over feature rows;
+//   awkward_hip_dispatch_choose           both in one launch: the workgroup
+//       that finishes the statistics last runs the inference (what the
+//       dispatcher and the batch runner call);
+//   awkward_hip_mlp_forward              the same network over feature rows;
 //   awkward_hip_mlp_train_step            forward + backward of a minibatch,
 //       one gradient row per workgroup (64 rows each);
 //   awkward_hip_mlp_adam                  sums the gradient rows in a fixed
@@ -106,16 +109,17 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long v) {
   return v;
 }
 
-// One workgroup per tile of kStatsLists lists of one item.
-__global__ __launch_bounds__(kStatsBlock) void stats_kernel(const StatsItem* __restrict__ items, int nitems,
-                                                            unsigned long long* __restrict__ acc) {
+// The statistics of one tile of kStatsLists lists of one item, added to its
+// slot's accumulators (one workgroup of kStatsBlock threads).
+__device__ inline void stats_tile(const StatsItem* __restrict__ items, int nitems, unsigned long long* __restrict__ acc,
+                                  int64_t tile) {
   constexpr int NW = kStatsBlock / kWave;
   __shared__ int s_item;
   __shared__ unsigned long long s_red[NW][13];
-  if (threadIdx.x == 0) s_item = find_stats_item(items, nitems, (int64_t)blockIdx.x);
+  if (threadIdx.x == 0) s_item = find_stats_item(items, nitems, tile);
   __syncthreads();
   const StatsItem it = items[s_item];
-  const int64_t l0 = ((int64_t)blockIdx.x - it.tile_begin) * kStatsLists;
+  const int64_t l0 = (tile - it.tile_begin) * kStatsLists;
   unsigned long long v[13];
 #pragma unroll
   for (int k = 0; k < 13; k++) v[k] = 0ull;
@@ -162,6 +166,12 @@ __global__ __launch_bounds__(kStatsBlock) void stats_kernel(const StatsItem* __r
   }
 }
 
+// One workgroup per tile.
+__global__ __launch_bounds__(kStatsBlock) void stats_kernel(const StatsItem* __restrict__ items, int nitems,
+                                                            unsigned long long* __restrict__ acc) {
+  stats_tile(items, nitems, acc, (int64_t)blockIdx.x);
+}
+
 // The 16 features of a slot from its accumulators (double arithmetic, then
 // rounded to float).
 __device__ inline void slot_features(const unsigned long long* a, int itemsize, int op, float* x) {
@@ -183,12 +193,14 @@ __device__ inline void slot_features(const unsigned long long* a, int itemsize, 
   x[15] = (float)((double)a[A_HEAVY] / nn);
 }
 
-// The network for one row, 64 threads: xraw (LDS, 16 raw features) ->
-// y (LDS, kOut outputs).  Returns (to every thread) whether every
+// The network for one row: xraw (LDS, 16 raw features) -> y (LDS, kOut
+// outputs), computed by the first kHidden threads of the workgroup (every
+// thread reaches the barriers).  Returns (to every thread) whether every
 // normalised feature lies inside the training range.
 __device__ inline bool mlp_eval(const float* __restrict__ P, const float* xraw, float* xn, float* h1, float* h2,
                                 float* y, int* s_in) {
   const int j = threadIdx.x;
+  const bool unit = j < kHidden;
   if (j == 0) *s_in = 1;
   __syncthreads();
   if (j < kFeatures) {
@@ -197,16 +209,20 @@ __device__ inline bool mlp_eval(const float* __restrict__ P, const float* xraw, 
     if (!(z >= P[kLo + j] && z <= P[kHi + j])) *s_in = 0;
   }
   __syncthreads();
-  float a = P[kB1 + j];
-  for (int k = 0; k < kFeatures; k++) a = fmaf(P[kW1 + k * kHidden + j], xn[k], a);
-  h1[j] = fmaxf(a, 0.0f);
+  if (unit) {
+    float a = P[kB1 + j];
+    for (int k = 0; k < kFeatures; k++) a = fmaf(P[kW1 + k * kHidden + j], xn[k], a);
+    h1[j] = fmaxf(a, 0.0f);
+  }
   __syncthreads();
-  a = P[kB2 + j];
-  for (int k = 0; k < kHidden; k++) a = fmaf(P[kW2 + k * kHidden + j], h1[k], a);
-  h2[j] = fmaxf(a, 0.0f);
+  if (unit) {
+    float a = P[kB2 + j];
+    for (int k = 0; k < kHidden; k++) a = fmaf(P[kW2 + k * kHidden + j], h1[k], a);
+    h2[j] = fmaxf(a, 0.0f);
+  }
   __syncthreads();
   if (j < kOut) {
-    a = P[kB3 + j];
+    float a = P[kB3 + j];
     for (int k = 0; k < kHidden; k++) a = fmaf(P[kW3 + k * kOut + j], h2[k], a);
     y[j] = a;
   }
@@ -214,28 +230,78 @@ __device__ inline bool mlp_eval(const float* __restrict__ P, const float* xraw, 
   return *s_in != 0;
 }
 
-// meta per slot: {itemsize, op, fallback variant, unused}
+struct InferSmem {
+  float x[kFeatures], xn[kFeatures], h1[kHidden], h2[kHidden], y[kOut];
+  int in;
+};
+
+// Features -> network -> choice of one slot from its accumulators `a`;
+// meta m: {itemsize, op, fallback variant, unused}.  Every thread of the
+// workgroup calls it (at least kHidden of them).
+__device__ inline void infer_slot(const unsigned long long* a, const int32_t* m, const float* __restrict__ P,
+                                  int slot, float* __restrict__ features, float* __restrict__ costs,
+                                  int32_t* __restrict__ choice, InferSmem& s) {
+  if (threadIdx.x == 0) slot_features(a, m[0], m[1], s.x);
+  __syncthreads();
+  const bool inside = mlp_eval(P, s.x, s.xn, s.h1, s.h2, s.y, &s.in);
+  if (threadIdx.x < kFeatures && features) features[slot * kFeatures + threadIdx.x] = s.x[threadIdx.x];
+  if (threadIdx.x < kOut && costs) costs[slot * kOut + threadIdx.x] = s.y[threadIdx.x];
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int v = 1; v < kVariantsN; v++)
+      if (s.y[v] < s.y[best]) best = v;
+    const bool empty = a[A_LISTS] == 0ull;
+    const bool use_fallback = !inside || empty;
+    choice[2 * slot] = use_fallback ? m[2] : best;
+    choice[2 * slot + 1] = use_fallback ? 1 : 0;
+  }
+}
+
+// One workgroup of 64 per slot, after stats_kernel.
 __global__ __launch_bounds__(64) void infer_kernel(const unsigned long long* __restrict__ acc,
                                                    const int32_t* __restrict__ meta, const float* __restrict__ P,
                                                    float* __restrict__ features, float* __restrict__ costs,
                                                    int32_t* __restrict__ choice) {
-  __shared__ float x[kFeatures], xn[kFeatures], h1[kHidden], h2[kHidden], y[kOut];
-  __shared__ int s_in;
+  __shared__ InferSmem s;
   const int slot = blockIdx.x;
-  const int32_t* m = meta + 4 * slot;
-  if (threadIdx.x == 0) slot_features(acc + (int64_t)slot * kAcc, m[0], m[1], x);
+  infer_slot(acc + (int64_t)slot * kAcc, meta + 4 * slot, P, slot, features, costs, choice, s);
+}
+
+// Statistics and inference in one launch: max(total_tiles, 1) workgroups
+// each add one tile's statistics, then count themselves done in the spare
+// accumulator word A_DONE of slot 0; the workgroup that counts last (every
+// other tile's atomics are then visible to it) copies each slot's
+// accumulators, clears them -- the accumulators start and end every launch
+// at zero, so no clearing launch or memset is needed, under a captured graph
+// too -- and runs the slot's inference.
+constexpr int A_DONE = kAcc - 1;
+static_assert(A_DONE >= A_BIN0 + kBins, "the completion count needs a spare accumulator word");
+static_assert(kStatsBlock >= kHidden, "the last workgroup runs the network");
+
+__global__ __launch_bounds__(kStatsBlock) void choose_kernel(const StatsItem* __restrict__ items, int nitems,
+                                                             int64_t total_tiles, unsigned long long* acc,
+                                                             const int32_t* __restrict__ meta, int nslots,
+                                                             const float* __restrict__ P, float* __restrict__ features,
+                                                             float* __restrict__ costs, int32_t* __restrict__ choice) {
+  __shared__ int s_last;
+  __shared__ unsigned long long s_a[kAcc];
+  __shared__ InferSmem s;
+  if (nitems > 0 && (int64_t)blockIdx.x < total_tiles) stats_tile(items, nitems, acc, (int64_t)blockIdx.x);
+  __threadfence();  // this workgroup's accumulator atomics before its count
   __syncthreads();
-  const bool inside = mlp_eval(P, x, xn, h1, h2, y, &s_in);
-  if (threadIdx.x < kFeatures && features) features[slot * kFeatures + threadIdx.x] = x[threadIdx.x];
-  if (threadIdx.x < kOut && costs) costs[slot * kOut + threadIdx.x] = y[threadIdx.x];
-  if (threadIdx.x == 0) {
-    int best = 0;
-    for (int v = 1; v < kVariantsN; v++)
-      if (y[v] < y[best]) best = v;
-    const bool empty = acc[(int64_t)slot * kAcc + A_LISTS] == 0ull;
-    const bool use_fallback = !inside || empty;
-    choice[2 * slot] = use_fallback ? m[2] : best;
-    choice[2 * slot + 1] = use_fallback ? 1 : 0;
+  if (threadIdx.x == 0)
+    s_last = __hip_atomic_fetch_add(acc + A_DONE, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+             (unsigned long long)gridDim.x - 1ull;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();  // every other workgroup's atomics before the reads below
+  for (int slot = 0; slot < nslots; slot++) {
+    unsigned long long* a = acc + (int64_t)slot * kAcc;
+    if (threadIdx.x < kAcc) s_a[threadIdx.x] = __hip_atomic_load(a + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (threadIdx.x < kAcc) __hip_atomic_store(a + threadIdx.x, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    infer_slot(s_a, meta + 4 * slot, P, slot, features, costs, choice, s);
+    __syncthreads();  // s_a and s are rewritten for the next slot
   }
 }
 
@@ -523,6 +589,23 @@ Error awkward_hip_dispatch_infer(const unsigned long long* acc, const int32_t* m
   if (nslots <= 0) return success();
   infer_kernel<<<(unsigned)nslots, 64, 0, stream()>>>(acc, meta, params, features, costs, choice);
   AKHIP_LAUNCH_CHECK("infer_kernel");
+  return success();
+}
+
+// Statistics and inference in one launch (the outputs of dispatch_stats
+// followed by dispatch_infer).  acc (nslots accumulator slots) must hold
+// zeros on entry and holds zeros again when the launch ends; launches that
+// share one acc must not overlap.
+Error awkward_hip_dispatch_choose(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* acc,
+                                  const int32_t* meta, int32_t nslots, const float* params, float* features,
+                                  float* costs, int32_t* choice) {
+  if (nslots <= 0) return success();
+  if (nitems < 0 || total_tiles < 0 || total_tiles > 0x7fffffffLL)
+    return failure("invalid statistics table for one launch", kSliceNone, kSliceNone, "awkward_amd/csrc/learned.hip");
+  const unsigned grid = total_tiles > 0 ? (unsigned)total_tiles : 1u;
+  choose_kernel<<<grid, kStatsBlock, 0, stream()>>>((const StatsItem*)items, nitems, total_tiles, acc, meta, nslots,
+                                                     params, features, costs, choice);
+  AKHIP_LAUNCH_CHECK("choose_kernel");
   return success();
 }
 

@@ -9,9 +9,10 @@ itself) and the dispatcher takes the cheapest:
 
 * features come from one device pass over the offsets (integer
   accumulators, so they are identical from run to run) and the network runs
-  on the device right after it (``awkward_hip_dispatch_infer``): the batch
-  runner enqueues stats -> inference -> ten predicated launches without a
-  host round trip, and a single-array call reads back one int;
+  in the same launch, in the workgroup that finishes the statistics last
+  (``awkward_hip_dispatch_choose``): the batch runner enqueues that launch
+  and ten predicated tile launches without a host round trip, and a
+  single-array call reads back one int;
 * a feature outside the range seen in training selects the autotuner table
   (bucketed exhaustive timings), else the fixed heuristic;
 * training data: the exhaustive sweep (``awkward_hip_dispatch_time_variants``)
@@ -102,12 +103,26 @@ class StatsPlan:
         self.table = (torch.from_numpy(np.array(rows, dtype=np.uint64).view(np.int64)).to(device)
                       if rows else torch.zeros(4, dtype=torch.int64, device=device))
         self.acc = torch.zeros(nslots * lay["acc_words"], dtype=torch.int64, device=device)
+        self._acc_clear = True  # the one-launch form leaves the accumulators at zero
 
     def run(self):
+        """The statistics alone: the accumulators stay in ``acc``."""
         set_stream()
         _check(_fn("awkward_hip_dispatch_reset", [P, I64])(self.acc.data_ptr(), self.nslots))
         _check(_fn("awkward_hip_dispatch_stats", [P, I32, I64, P])(self.table.data_ptr(), self.nitems,
                                                                    self.total_tiles, self.acc.data_ptr()))
+        self._acc_clear = False
+
+    def choose(self, meta, params, features=None, costs=None, choice=None):
+        """Statistics and inference in one launch (``awkward_hip_dispatch_choose``)."""
+        set_stream()
+        if not self._acc_clear:
+            _check(_fn("awkward_hip_dispatch_reset", [P, I64])(self.acc.data_ptr(), self.nslots))
+        _check(_fn("awkward_hip_dispatch_choose", [P, I32, I64, P, P, I32, P, P, P, P])(
+            self.table.data_ptr(), self.nitems, self.total_tiles, self.acc.data_ptr(), meta.data_ptr(), self.nslots,
+            params.data_ptr(), 0 if features is None else features.data_ptr(),
+            0 if costs is None else costs.data_ptr(), choice.data_ptr()))
+        self._acc_clear = True
 
 
 def infer(acc, meta, nslots, params, features=None, costs=None, choice=None):
@@ -215,8 +230,7 @@ class Dispatcher:
         return plan, meta, choice
 
     def run_device(self, plan, meta, choice, costs=None):
-        plan.run()
-        infer(plan.acc, meta, plan.nslots, self.model.device_params(plan.acc.device), costs=costs, choice=choice)
+        plan.choose(meta, self.model.device_params(plan.acc.device), costs=costs, choice=choice)
 
     def choose(self, op: str | int, offsets, n: int, N: int, itemsize: int) -> int:
         code = OP_CODE[op] if isinstance(op, str) else int(op)

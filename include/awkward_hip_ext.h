@@ -354,6 +354,10 @@ struct Error awkward_hip_batch_argsort_float64(const void* items, int32_t nitems
      0 sum, 1 min/max, 2 argmin/argmax, 3 batched sum+min+argmin; writes
      features (16 per slot), costs (11 per slot) -- either may be NULL -- and
      choice (2 per slot: tile, 1 if the fallback was taken).
+   choose: stats then infer in one launch (the workgroup that finishes the
+     statistics last runs the network); acc must hold zeros on entry and
+     holds zeros again when the launch ends; launches sharing one acc must not
+     overlap.
    mlp_forward: rows of raw features -> 11 outputs (+ inside-range flags).
    mlp_train_step / mlp_adam: one minibatch's gradients (one row per
      workgroup of 64) and the Adam step over their fixed-order sum.
@@ -363,6 +367,7 @@ int32_t awkward_hip_dispatch_layout(int32_t what);
 struct Error awkward_hip_dispatch_reset(unsigned long long* acc, int64_t nslots);
 struct Error awkward_hip_dispatch_stats(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* acc);
 struct Error awkward_hip_dispatch_infer(const unsigned long long* acc, const int32_t* meta, int32_t nslots, const float* params, float* features, float* costs, int32_t* choice);
+struct Error awkward_hip_dispatch_choose(const void* items, int32_t nitems, int64_t total_tiles, unsigned long long* acc, const int32_t* meta, int32_t nslots, const float* params, float* features, float* costs, int32_t* choice);
 struct Error awkward_hip_mlp_forward(const float* X, int64_t rows, const float* params, float* Y, int32_t* inside);
 struct Error awkward_hip_mlp_train_step(const float* X, const float* T, const float* ow, int64_t R, int64_t pa, int64_t pb, int64_t row0, int64_t bsz, const float* params, float* grads, float* losses);
 struct Error awkward_hip_mlp_adam(float* params, float* m, float* v, const float* grads, const float* losses, int32_t nparts, float lr, float beta1, float beta2, float eps, int64_t step, float* loss_acc);

@@ -200,6 +200,105 @@ def test_infer_equals_forward_and_takes_fallback():
             np.testing.assert_array_equal(c[:, 1], [0, 0])
 
 
+def _two_launch(plan, meta, params):
+    """stats_kernel then infer_kernel: features, costs and choice."""
+    nslots = plan.nslots
+    feats = torch.full((nslots * 16,), -1.0, dtype=torch.float32, device="cuda")
+    costs = torch.full((nslots * 11,), -1.0, dtype=torch.float32, device="cuda")
+    choice = torch.full((2 * nslots,), -1, dtype=torch.int32, device="cuda")
+    plan.run()
+    learned.infer(plan.acc, meta, nslots, params, features=feats, costs=costs, choice=choice)
+    return feats.cpu().numpy(), costs.cpu().numpy(), choice.cpu().numpy()
+
+
+def _one_launch(plan, meta, params, out=None):
+    nslots = plan.nslots
+    if out is None:
+        out = (torch.full((nslots * 16,), -1.0, dtype=torch.float32, device="cuda"),
+               torch.full((nslots * 11,), -1.0, dtype=torch.float32, device="cuda"),
+               torch.full((2 * nslots,), -1, dtype=torch.int32, device="cuda"))
+    plan.choose(meta, params, features=out[0], costs=out[1], choice=out[2])
+    return out
+
+
+@gpu
+@pytest.mark.parametrize("case", ["many_tiles", "one_tile", "empty_slot", "no_lists", "fallback"])
+def test_one_launch_choice_equals_stats_then_infer(case):
+    """awkward_hip_dispatch_choose (statistics whose last workgroup runs the
+    network) gives the features, costs and choice of the two-kernel form bit
+    for bit, leaves its accumulators at zero, and so gives the same answer
+    when called again (and after the two-kernel form left its sums behind)."""
+    lay, k = _lay()
+    rng = np.random.default_rng(21)
+    parts = [rng.poisson(4, 300000), rng.poisson(60, 9000), np.minimum(rng.zipf(1.6, 70000), 3000),
+             rng.poisson(2, 5)]
+    offs = [_offsets(c) for c in parts]
+    dev = offs[0].device
+    entries = {"many_tiles": [(offs[0], 300000, 1), (offs[1], 9000, 0), (offs[2], 70000, 1)],
+               "one_tile": [(offs[3], 5, 0)],
+               "empty_slot": [(offs[1], 9000, 0), (offs[2], 70000, 2)],
+               "no_lists": [(offs[3], 0, 0)],
+               "fallback": [(offs[0], 300000, 0), (offs[2], 70000, 1)]}[case]
+    nslots = {"many_tiles": 2, "one_tile": 1, "empty_slot": 3, "no_lists": 1, "fallback": 2}[case]
+    p = _random_params(rng, lay, k, **({"lo": 5.0, "hi": 6.0} if case == "fallback" else {}))
+    params = torch.from_numpy(p).cuda()
+    meta = torch.tensor([[8, s % 4, 3 + s, 0] for s in range(nslots)], dtype=torch.int32).reshape(-1).cuda()
+    plan = learned.StatsPlan(entries, nslots, dev)
+    for _ in range(2):
+        f, c, ch = (t.cpu().numpy() for t in _one_launch(plan, meta, params))
+        assert not plan.acc.any().item()
+        fr, cr, chr_ = _two_launch(learned.StatsPlan(entries, nslots, dev), meta, params)
+        np.testing.assert_array_equal(f, fr)
+        np.testing.assert_array_equal(c, cr)
+        np.testing.assert_array_equal(ch, chr_)
+    # the two-kernel form on the same plan leaves sums behind; the next one
+    # launch clears them first
+    _two_launch(plan, meta, params)
+    assert plan.acc.any().item() == (case not in ("no_lists",))
+    f2, c2, ch2 = (t.cpu().numpy() for t in _one_launch(plan, meta, params))
+    np.testing.assert_array_equal(f2, f)
+    np.testing.assert_array_equal(ch2, ch)
+    ch = ch.reshape(nslots, 2)
+    if case == "fallback":
+        np.testing.assert_array_equal(ch, [[3, 1], [4, 1]])
+    if case == "empty_slot":
+        assert tuple(ch[1]) == (4, 1)  # no lists: the fallback
+    if case == "no_lists":
+        assert tuple(ch[0]) == (3, 1)
+
+
+@gpu
+def test_one_launch_choice_in_a_graph():
+    """The one-launch choice captured in a hipGraph: the accumulators end
+    every replay at zero, so every replay (eager library work and
+    synchronisations in between) equals the eager answer."""
+    from awkward_amd.fused import Graph
+
+    lay, k = _lay()
+    rng = np.random.default_rng(22)
+    parts = [rng.poisson(30, 200000), np.minimum(rng.zipf(1.5, 50000), 5000)]
+    offs = [_offsets(c) for c in parts]
+    p = _random_params(rng, lay, k)
+    params = torch.from_numpy(p).cuda()
+    meta = torch.tensor([4, 0, 2, 0, 8, 3, 5, 0], dtype=torch.int32, device="cuda")
+    plan = learned.StatsPlan([(offs[0], 200000, 0), (offs[1], 50000, 1)], 2, offs[0].device)
+    out = _one_launch(plan, meta, params)
+    torch.cuda.synchronize()
+    eager = [t.clone() for t in out]
+    g = Graph(lambda: _one_launch(plan, meta, params, out))
+    other = learned.StatsPlan([(offs[1], 50000, 0)], 1, offs[0].device)
+    for _ in range(3):
+        for t in out:
+            t.fill_(-7)
+        _two_launch(other, meta[:4], params)  # eager work between the replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(out, eager):
+            assert torch.equal(a, b)
+        assert not plan.acc.any().item()
+
+
 @gpu
 def test_train_step_gradient_matches_autograd():
     lay, k = _lay()
