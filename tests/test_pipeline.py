@@ -133,9 +133,9 @@ def test_argsort_sort_api(backends, ascending):
 @pytest.mark.parametrize("case", ["poisson", "long_lists", "offset_start", "unaligned_mask", "all_false", "all_true",
                                   "empty_lists"])
 def test_bool_to_index_kernel(case):
-    """awkward_hip_bool_to_index (tiles of 1024 lists whose mask bytes are
+    """awkward_hip_bool_to_index (tiles of 3072 lists whose mask bytes are
     staged in LDS, or read from global memory when a tile's span exceeds the
-    8 KiB stage) against numpy: the local positions of the true values per
+    12 KiB stage) against numpy: the local positions of the true values per
     list and their offsets (the reference's _slicing.py:476-530 steps)."""
     import torch
 
